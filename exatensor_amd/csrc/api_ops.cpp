@@ -1,0 +1,1188 @@
+// talsh.h task objects and tensor operations (place/discard, init, scale,
+// copy, add, contract) on MI355X GPUs.
+//
+// Dispatch and coherence semantics follow TALSH/talshc.cpp (tasks :1935-2597,
+// place/discard :3318-3560, init/scale :3566-4002, copy :4530-4789, add
+// :4795-5067, contract :5073-5385) and the copy-control meanings of
+// TALSH/tensor_algebra.h:119-203:
+//   D discard the argument everywhere after the task; M keep it only on the
+//   executing device; T keep it only on its source device; K keep both.
+// The GPU work of one task runs in order on one stream of the pool; tasks that
+// touch the same device image are ordered through per-image events.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "kernels.hpp"
+#include "runtime_impl.hpp"
+#include "talsh_amd.h"
+
+namespace xt {
+
+int launch_axpy_dev(void *d, const void *l, const void *s, int kind, size_t vol, double are, double aim, bool lconj,
+                    bool sconj, bool acc, hipStream_t st);
+
+namespace {
+
+Task *task_of(const talsh_task_t *t) { return t ? static_cast<Task *>(t->task_p) : nullptr; }
+
+int new_task(talsh_task_t *tt, Device *d, int kind, unsigned coh, int nargs)
+{
+  Task *k = new Task();
+  k->dev_flat = d->flat_id;
+  k->stream = d->streams[d->next_stream.fetch_add(1) % kStreamsPerDevice];
+  bool ok = true;
+  for (hipEvent_t *e : {&k->ev_start, &k->ev_comput, &k->ev_output, &k->ev_finish, &k->ev_mm0, &k->ev_mm1})
+    ok = ok && hipEventCreate(e) == hipSuccess;
+  if (!ok) {
+    delete k;
+    return TALSH_FAILURE;
+  }
+  k->status = TALSH_TASK_SCHEDULED;
+  tt->task_p = k;
+  tt->task_error = -1;
+  tt->dev_kind = DEV_AMD_GPU;
+  tt->data_kind = kind;
+  tt->coherence = (int)coh;
+  tt->num_args = nargs;
+  tt->data_vol = 0;
+  tt->flops = 0;
+  tt->exec_time = 0;
+  return 0;
+}
+
+void destroy_task_events(Task *k)
+{
+  for (hipEvent_t e : {k->ev_start, k->ev_comput, k->ev_output, k->ev_finish, k->ev_mm0, k->ev_mm1})
+    if (e) (void)hipEventDestroy(e);
+}
+
+// Prepared operand of an operation.
+struct Arg {
+  talsh_tens_t *t = nullptr;
+  unsigned coh = COPY_M;
+  bool output = false, need_input = true;
+  void *src = nullptr;       // source image body
+  int src_dev = -1;
+  void *exec = nullptr;      // body on the executing GPU
+  bool created = false;      // exec image allocated for this task
+  talsh_dev_rsc_t new_rsc{};
+  int kind = NO_TYPE;
+  void *attach = nullptr;    // caller memory to use as the executing-device image
+};
+
+// Chooses the source image (prefer one on the executing device) and allocates
+// the executing-device image if needed. No GPU work is enqueued here.
+int prepare(Arg &a, Device *d)
+{
+  if (!a.t || talshTensorIsEmpty(a.t) == YEP) return TALSH_OBJECT_IS_EMPTY;
+  int idx = -1;
+  for (int i = 0; i < a.t->ndev; ++i)
+    if (a.t->dev_rsc[i].dev_id == d->flat_id && a.t->avail[i] == YEP) { idx = i; break; }
+  if (idx < 0)
+    for (int i = 0; i < a.t->ndev; ++i)
+      if (a.t->avail[i] == YEP) { idx = i; break; }
+  if (idx < 0) return a.t->ndev == 0 ? TALSH_OBJECT_IS_EMPTY : TALSH_IN_PROGRESS;
+  a.kind = a.t->data_kind[idx];
+  a.src = a.t->dev_rsc[idx].gmem_p;
+  a.src_dev = a.t->dev_rsc[idx].dev_id;
+  if (a.src_dev == d->flat_id) {
+    a.exec = a.src;
+    return 0;
+  }
+  if (a.attach) {
+    a.new_rsc.dev_id = d->flat_id;
+    a.new_rsc.gmem_p = a.attach;
+    a.new_rsc.buf_entry = -1;
+    a.new_rsc.mem_attached = 1;
+  } else {
+    const size_t bytes = tensor_volume(a.t) * (size_t)kind_size(a.kind);
+    const int e = image_alloc(d->flat_id, bytes, &a.new_rsc);
+    if (e) return e;
+  }
+  a.exec = a.new_rsc.gmem_p;
+  a.created = true;
+  return 0;
+}
+
+void unprepare(Arg &a)
+{
+  if (a.created) image_release(a.new_rsc, nullptr);
+  a.created = false;
+}
+
+// Temporary device buffers of one task (released at completion).
+struct Temps {
+  Device *d;
+  std::vector<talsh_dev_rsc_t> bufs;
+  int alloc(size_t bytes, void **p)
+  {
+    talsh_dev_rsc_t r;
+    int e = image_alloc(d->flat_id, bytes, &r);
+    if (e) return e;
+    bufs.push_back(r);
+    *p = r.gmem_p;
+    return 0;
+  }
+  void rollback()
+  {
+    for (auto &r : bufs) image_release(r, nullptr);
+    bufs.clear();
+  }
+};
+
+// Enqueues input transfers and registers the coherence actions for completion.
+void enqueue_inputs(Task *k, std::vector<Arg> &args)
+{
+  for (Arg &a : args) {
+    image_wait(k->stream, a.src);
+    if (a.exec != a.src) image_wait(k->stream, a.exec);
+    if (a.created && a.need_input) {
+      const size_t bytes = tensor_volume(a.t) * (size_t)kind_size(a.kind);
+      (void)hipMemcpyAsync(a.exec, a.src, bytes, a.src_dev == 0 ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
+                           k->stream);
+      if (Device *d = gpu_by_flat(k->dev_flat)) d->stats.traffic_in += (double)bytes;
+    }
+  }
+}
+
+void enqueue_outputs(Task *k, std::vector<Arg> &args)
+{
+  for (Arg &a : args) {
+    if (!a.output || !a.created) continue;
+    if (a.coh == COPY_T || a.coh == COPY_K) {
+      const size_t bytes = tensor_volume(a.t) * (size_t)kind_size(a.kind);
+      (void)hipMemcpyAsync(a.src, a.exec, bytes, a.src_dev == 0 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
+                           k->stream);
+      if (Device *d = gpu_by_flat(k->dev_flat)) d->stats.traffic_out += (double)bytes;
+    }
+  }
+}
+
+// Registers the new images with their tensors and marks the images that the
+// task writes or will discard as unavailable until completion (inputs that
+// stay valid remain usable by concurrent tasks).
+void commit_args(Task *k, std::vector<Arg> &args, int exec_flat)
+{
+  for (Arg &a : args) {
+    Task::ArgFinal f;
+    f.tens = a.t;
+    f.src = a.src;
+    f.exec = a.exec;
+    f.coh = a.coh;
+    f.output = a.output;
+    f.created = a.created;
+    k->finals.push_back(f);
+    if (a.created) {
+      const int ei = tensor_add_image(a.t, a.new_rsc, a.kind);
+      if (ei >= 0) a.t->avail[ei] = NOPE;
+    }
+    const bool lock_src = a.output || a.coh == COPY_D || (a.coh == COPY_M && a.src_dev != exec_flat);
+    if (lock_src)
+      for (int i = 0; i < a.t->ndev; ++i)
+        if (a.t->dev_rsc[i].gmem_p == a.src) a.t->avail[i] = NOPE;
+  }
+}
+
+}  // namespace
+
+static void remove_by_ptr(talsh_tens_t *t, void *p)
+{
+  for (int i = 0; i < t->ndev; ++i)
+    if (t->dev_rsc[i].gmem_p == p) { tensor_remove_image(t, i, nullptr); return; }
+}
+
+// Completion: coherence actions, temporaries, statistics.
+int task_finalize(talsh_task_t *tt, int status)
+{
+  Task *k = task_of(tt);
+  if (!k || k->finalized) return 0;
+  for (size_t a = 0; a < k->finals.size(); ++a) {
+    talsh_tens_t *t = k->finals[a].tens;
+    const Task::ArgFinal &p = k->finals[a];
+    if (!t || !t->shape_p) continue;
+    for (int i = 0; i < t->ndev; ++i)
+      if (t->dev_rsc[i].gmem_p == p.src || t->dev_rsc[i].gmem_p == p.exec) t->avail[i] = YEP;
+    if (status != TALSH_TASK_COMPLETED) {
+      if (p.created) remove_by_ptr(t, p.exec);  // failed task: drop the half-made image
+      continue;
+    }
+    bool keep_src = true, keep_exec = true;
+    if (p.created) {
+      switch (p.coh) {
+      case COPY_D: keep_src = keep_exec = false; break;
+      case COPY_M: keep_src = false; break;
+      case COPY_T: keep_exec = false; break;
+      default: break;
+      }
+      if (p.output && p.coh == COPY_M) keep_src = false;
+    } else if (p.coh == COPY_D) {
+      keep_src = keep_exec = false;
+    }
+    if (p.output) {
+      // every image other than the freshly written ones is stale
+      for (int i = t->ndev - 1; i >= 0; --i) {
+        void *q = t->dev_rsc[i].gmem_p;
+        const bool fresh = (q == p.exec) || (q == p.src && p.created && (p.coh == COPY_T || p.coh == COPY_K));
+        if (!fresh) tensor_remove_image(t, i, nullptr);
+      }
+      if (!p.created && p.coh == COPY_D) remove_by_ptr(t, p.exec);
+      if (p.created && !keep_exec) remove_by_ptr(t, p.exec);
+      if (p.created && !keep_src) remove_by_ptr(t, p.src);
+    } else {
+      if (!keep_exec && p.exec != p.src) remove_by_ptr(t, p.exec);
+      if (!keep_src) remove_by_ptr(t, p.src);
+      if (!p.created && !keep_exec) remove_by_ptr(t, p.exec);
+    }
+  }
+  for (auto &f : k->frees) pending_free(f);
+  k->frees.clear();
+  Device *d = gpu_by_flat(k->dev_flat);
+  if (d) {
+    if (status == TALSH_TASK_COMPLETED) {
+      d->stats.tasks_completed++;
+      d->stats.flops += k->flops;
+    } else {
+      d->stats.tasks_failed++;
+    }
+  }
+  k->status = status;
+  k->finalized = true;
+  tt->task_error = status == TALSH_TASK_COMPLETED ? 0 : 1;
+  float ms = 0;
+  if (status == TALSH_TASK_COMPLETED && hipEventElapsedTime(&ms, k->ev_start, k->ev_finish) == hipSuccess)
+    tt->exec_time = ms * 1e-3;
+  return 0;
+}
+
+namespace {
+
+// Shared scheduling skeleton: creates/validates the task handle, prepares the
+// operands (allocations first, so TRY_LATER leaves everything unchanged),
+// then runs `body` on the stream and finishes the task.
+template <typename Body>
+int run_task(talsh_task_t *user, Device *d, unsigned coh, std::vector<Arg> &args, int kind, double flops,
+             Body body)
+{
+  Engine &E = engine();
+  if (!E.on) return TALSH_NOT_INITIALIZED;
+  talsh_task_t local;
+  talsh_task_t *tt = user ? user : &local;
+  if (!user) talshTaskClean(&local);
+  else if (talshTaskIsEmpty(user) != YEP) return TALSH_OBJECT_NOT_EMPTY;
+  (void)hipSetDevice(d->ordinal);
+  const int nargs = (int)args.size();
+  for (int a = 0; a < nargs; ++a) {
+    args[a].coh = argument_coherence_get_value(coh, (unsigned)nargs, (unsigned)a);
+    if (args[a].coh > 3) args[a].coh = COPY_M;
+  }
+  int e = 0;
+  int prepared = 0;
+  for (; prepared < nargs && !e; ++prepared) e = prepare(args[prepared], d);
+  if (!e) {
+    for (const Arg &a : args)
+      if (a.kind != kind && kind != NO_TYPE) e = TALSH_INVALID_ARGS;
+  }
+  Temps temps{d, {}};
+  if (!e) e = body.allocate(temps);
+  if (e) {
+    temps.rollback();
+    for (int a = 0; a < prepared; ++a) unprepare(args[a]);
+    if (e == TRY_LATER || e == DEVICE_UNABLE) d->stats.tasks_deferred++;
+    return e;
+  }
+  e = new_task(tt, d, kind, coh, nargs);
+  if (e) {
+    temps.rollback();
+    for (Arg &a : args) unprepare(a);
+    return e;
+  }
+  Task *k = task_of(tt);
+  k->flops = flops;
+  for (int a = 0; a < nargs; ++a) {
+    tt->tens_args[a].tens_p = args[a].t;
+    tt->tens_args[a].source_image = 0;
+  }
+  d->stats.tasks_submitted++;
+  (void)hipEventRecord(k->ev_start, k->stream);
+  enqueue_inputs(k, args);
+  (void)hipEventRecord(k->ev_comput, k->stream);
+  e = body.enqueue(k, d);
+  (void)hipEventRecord(k->ev_output, k->stream);
+  enqueue_outputs(k, args);
+  (void)hipEventRecord(k->ev_finish, k->stream);
+  for (Arg &a : args) {
+    image_touch(k->stream, a.exec);
+    if (a.src != a.exec) image_touch(k->stream, a.src);
+  }
+  for (auto &r : temps.bufs) k->frees.push_back(PendingFree{r.dev_id, r.buf_entry, nullptr});
+  commit_args(k, args, d->flat_id);
+  tt->flops = flops;
+  if (e) {
+    (void)hipStreamSynchronize(k->stream);
+    task_finalize(tt, TALSH_TASK_ERROR);
+    if (!user) talshTaskDestruct(&local);
+    return TALSH_FAILURE;
+  }
+  if (!user) {
+    int st = 0;
+    e = talshTaskWait(&local, &st);
+    if (!e && st != TALSH_TASK_COMPLETED) e = TALSH_TASK_ERROR;
+    const int e2 = talshTaskDestruct(&local);
+    if (!e) e = e2;
+  }
+  return e;
+}
+
+Device *exec_device(int dev_id, int dev_kind, int *err)
+{
+  Engine &E = engine();
+  *err = 0;
+  if (!E.on) { *err = TALSH_NOT_INITIALIZED; return nullptr; }
+  const int flat = resolve_device(dev_id, dev_kind);
+  if (flat == 0) { *err = TALSH_NOT_AVAILABLE; return nullptr; }  // tensor operations execute on GPUs
+  Device *d = flat > 0 ? gpu_by_flat(flat) : nullptr;
+  if (!d) *err = (E.gpus.empty() ? TALSH_NOT_AVAILABLE : TALSH_INVALID_ARGS);
+  return d;
+}
+
+int tensor_kind(const talsh_tens_t *t)
+{
+  if (!t || !t->shape_p || t->ndev <= 0) return NO_TYPE;
+  return t->data_kind[0];
+}
+
+std::vector<int> dims_of(const talsh_tens_t *t)
+{
+  return std::vector<int>(t->shape_p->dims, t->shape_p->dims + t->shape_p->num_dim);
+}
+
+struct NoTemps {
+  int allocate(Temps &) { return 0; }
+};
+
+}  // namespace
+
+// ---- transposition helper used by the planners --------------------------------
+// o2n (0-based) that reorders labels `from` into the order `to`.
+static std::vector<int> reorder(const std::vector<int> &from, const std::vector<int> &to)
+{
+  std::vector<int> o2n(from.size());
+  for (size_t i = 0; i < from.size(); ++i)
+    o2n[i] = (int)(std::find(to.begin(), to.end(), from[i]) - to.begin());
+  return o2n;
+}
+
+static bool is_identity(const std::vector<int> &p)
+{
+  for (size_t i = 0; i < p.size(); ++i)
+    if (p[i] != (int)i) return false;
+  return true;
+}
+
+static std::vector<int> concat(const std::vector<int> &a, const std::vector<int> &b)
+{
+  std::vector<int> c(a);
+  c.insert(c.end(), b.begin(), b.end());
+  return c;
+}
+
+// Contraction plan: how the operands are matricized for the GEMM.
+struct ContractPlan {
+  int kase = 0;                       // 0 scalar product, 1 scaled add, 2 full, 3 GEMM
+  bool swap = false;                  // GEMM computes D^T = R^T L^T
+  std::vector<int> l_o2n, r_o2n;      // transposes of L / R (empty: none)
+  std::vector<int> d_in_o2n, d_out_o2n;  // D -> Dmat and back (empty: none)
+  std::vector<int> lmat_dims, rmat_dims, dmat_dims;
+  bool a_kfast = true, b_kfast = true;
+  size_t m = 1, n = 1, k = 1;
+  double bytes_moved = 0;
+};
+
+// Heuristic plan: among the index orders that keep an operand in place
+// (contracted indices in L's or R's order, free indices in their own or D's
+// order, optionally computing D^T), pick the one moving the fewest bytes.
+static int plan_contraction(const std::vector<int> &dig, int dr, int lr, int rr, const std::vector<int> &dd,
+                            const std::vector<int> &ld, const std::vector<int> &rd, int es, bool acc,
+                            int variant, ContractPlan &P, std::vector<ContractPlan> *all)
+{
+  std::vector<int> llab(lr), rlab(rr), dlab(dr);
+  std::vector<int> ext(256, 0);
+  for (int q = 0; q < dr; ++q) { dlab[q] = q + 1; ext[q + 1] = dd[q]; }
+  for (int i = 0; i < lr; ++i) {
+    llab[i] = dig[i] > 0 ? dig[i] : 100 + i + 1;
+    if (dig[i] < 0) ext[100 + i + 1] = ld[i];
+  }
+  for (int j = 0; j < rr; ++j) rlab[j] = dig[lr + j] > 0 ? dig[lr + j] : 100 + (-dig[lr + j]);
+  std::vector<int> Kl, Kr, Ml, Nr, Md, Nd;
+  for (int i = 0; i < lr; ++i) (dig[i] < 0 ? Kl : Ml).push_back(llab[i]);
+  for (int j = 0; j < rr; ++j) (dig[lr + j] < 0 ? Kr : Nr).push_back(rlab[j]);
+  for (int lab : Ml)
+    if (std::find(Nr.begin(), Nr.end(), lab) != Nr.end()) return TALSH_INVALID_ARGS;  // hyper-index
+  for (int q = 0; q < dr; ++q) {
+    if (std::find(Ml.begin(), Ml.end(), dlab[q]) != Ml.end()) Md.push_back(dlab[q]);
+    else Nd.push_back(dlab[q]);
+  }
+  auto vol = [&](const std::vector<int> &labs) {
+    size_t v = 1;
+    for (int l : labs) v *= (size_t)ext[l];
+    return v;
+  };
+  for (int i = 0; i < lr; ++i) ext[llab[i]] = ld[i];
+  const size_t vl = vol(llab), vr = vol(rlab), vd = vol(dlab);
+  std::vector<ContractPlan> cands;
+  for (int ko = 0; ko < 2; ++ko)
+    for (int mo = 0; mo < 2; ++mo)
+      for (int no = 0; no < 2; ++no)
+        for (int sw = 0; sw < 2; ++sw) {
+          const std::vector<int> &K = ko ? Kr : Kl, &M = mo ? Md : Ml, &N = no ? Nd : Nr;
+          ContractPlan c;
+          c.kase = 3;
+          c.swap = sw;
+          c.m = vol(M); c.n = vol(N); c.k = vol(K);
+          double cost = 0;
+          const auto KM = concat(K, M), MK = concat(M, K), KN = concat(K, N), NK = concat(N, K);
+          bool lk = true, rk = true;
+          if (llab == KM) lk = true;
+          else if (llab == MK) lk = false;
+          else { c.l_o2n = reorder(llab, KM); cost += 2.0 * vl * es; }
+          if (rlab == KN) rk = true;
+          else if (rlab == NK) rk = false;
+          else { c.r_o2n = reorder(rlab, KN); cost += 2.0 * vr * es; }
+          const auto DT = sw ? concat(N, M) : concat(M, N);
+          if (dlab != DT) {
+            c.d_in_o2n = reorder(dlab, DT);
+            c.d_out_o2n = reorder(DT, dlab);
+            cost += (acc ? 4.0 : 2.0) * vd * es;
+            for (int lab : DT) c.dmat_dims.push_back(ext[lab]);
+          }
+          if (!sw) { c.a_kfast = lk; c.b_kfast = rk; }
+          else { c.a_kfast = rk; c.b_kfast = lk; }
+          c.bytes_moved = cost;
+          cands.push_back(c);
+        }
+  // deterministic preference on ties: fewer bytes, then k-fast operands, then no swap
+  auto score = [](const ContractPlan &c) {
+    return c.bytes_moved * 4.0 + (c.a_kfast ? 0 : 1) + (c.b_kfast ? 0 : 1) + (c.swap ? 0.5 : 0);
+  };
+  std::stable_sort(cands.begin(), cands.end(),
+                   [&](const ContractPlan &a, const ContractPlan &b) { return score(a) < score(b); });
+  if (all) *all = cands;
+  if (variant >= 0 && variant < (int)cands.size()) P = cands[variant];
+  else P = cands.front();
+  return 0;
+}
+
+}  // namespace xt
+
+using namespace xt;
+
+// Selected plan variant for the next contraction (-1: heuristic); set by the
+// plan selector extension.
+static int g_plan_variant = -1;
+static int g_gemm_tile = -1;
+
+extern "C" {
+
+int talshAmdSetPlanVariant(int variant, int tile)
+{
+  g_plan_variant = variant;
+  g_gemm_tile = tile;
+  return 0;
+}
+
+// ============================================================================ tasks
+int talshTaskCreate(talsh_task_t **t)
+{
+  if (!t) return TALSH_INVALID_ARGS;
+  *t = (talsh_task_t *)std::malloc(sizeof(talsh_task_t));
+  if (!*t) return TRY_LATER;
+  return talshTaskClean(*t);
+}
+
+int talshTaskClean(talsh_task_t *t)
+{
+  if (!t) return TALSH_INVALID_ARGS;
+  t->task_p = nullptr;
+  t->task_error = -1;
+  t->dev_kind = DEV_NULL;
+  t->data_kind = NO_TYPE;
+  t->coherence = -1;
+  t->num_args = 0;
+  for (auto &a : t->tens_args) { a.tens_p = nullptr; a.source_image = -1; }
+  t->data_vol = 0;
+  t->flops = 0;
+  t->exec_time = 0;
+  return 0;
+}
+
+int talshTaskIsEmpty(const talsh_task_t *t)
+{
+  if (!t) return TALSH_INVALID_ARGS;
+  return t->task_p == nullptr ? YEP : NOPE;
+}
+
+int talshTaskDestruct(talsh_task_t *t)
+{
+  if (!t) return TALSH_INVALID_ARGS;
+  Task *k = task_of(t);
+  if (k) {
+    if (!k->finalized) {
+      const hipError_t q = hipEventQuery(k->ev_finish);
+      if (q == hipErrorNotReady) return TALSH_IN_PROGRESS;
+      task_finalize(t, q == hipSuccess ? TALSH_TASK_COMPLETED : TALSH_TASK_ERROR);
+    }
+    destroy_task_events(k);
+    delete k;
+  }
+  return talshTaskClean(t);
+}
+
+int talshTaskDestroy(talsh_task_t *t)
+{
+  if (!t) return TALSH_INVALID_ARGS;
+  const int e = talshTaskDestruct(t);
+  if (e == TALSH_IN_PROGRESS) return e;
+  std::free(t);
+  return e;
+}
+
+int talshTaskDevId(talsh_task_t *t, int *dev_kind)
+{
+  Task *k = task_of(t);
+  if (!k) { if (dev_kind) *dev_kind = DEV_NULL; return DEV_NULL; }
+  int kind = DEV_NULL;
+  const int n = decode_device_id(k->dev_flat, &kind);
+  if (dev_kind) { *dev_kind = kind; return n; }
+  return k->dev_flat;
+}
+int talshTaskDevId_(talsh_task_t *t, int *dev_kind) { return talshTaskDevId(t, dev_kind); }
+
+int talshTaskArgCoherence(const talsh_task_t *t) { return t ? t->coherence : -1; }
+
+const talshTensArg_t *talshTaskTensArgs(const talsh_task_t *t, int *num_args)
+{
+  if (!t) { if (num_args) *num_args = 0; return nullptr; }
+  if (num_args) *num_args = t->num_args;
+  return t->tens_args;
+}
+
+int talshTaskStatus(talsh_task_t *t)
+{
+  if (!t) return TALSH_INVALID_ARGS;
+  Task *k = task_of(t);
+  if (!k) return TALSH_TASK_EMPTY;
+  if (k->finalized) return k->status;
+  const hipError_t q = hipEventQuery(k->ev_finish);
+  if (q == hipErrorNotReady) {
+    if (hipEventQuery(k->ev_output) == hipSuccess) return TALSH_TASK_OUTPUT_READY;
+    if (hipEventQuery(k->ev_comput) == hipSuccess) return TALSH_TASK_INPUT_READY;
+    if (hipEventQuery(k->ev_start) == hipSuccess) return TALSH_TASK_STARTED;
+    return TALSH_TASK_SCHEDULED;
+  }
+  task_finalize(t, q == hipSuccess ? TALSH_TASK_COMPLETED : TALSH_TASK_ERROR);
+  return k->status;
+}
+
+int talshTaskComplete(talsh_task_t *t, int *stats, int *ierr)
+{
+  if (ierr) *ierr = 0;
+  if (!t) { if (ierr) *ierr = TALSH_INVALID_ARGS; return NOPE; }
+  const int st = talshTaskStatus(t);
+  if (stats) *stats = st;
+  if (st == TALSH_TASK_EMPTY) { if (ierr) *ierr = TALSH_OBJECT_IS_EMPTY; return NOPE; }
+  if (st == TALSH_TASK_ERROR) { if (ierr) *ierr = TALSH_TASK_ERROR; return YEP; }
+  return st == TALSH_TASK_COMPLETED ? YEP : NOPE;
+}
+
+int talshTaskWait(talsh_task_t *t, int *stats)
+{
+  if (!t) return TALSH_INVALID_ARGS;
+  Task *k = task_of(t);
+  if (!k) { if (stats) *stats = TALSH_TASK_EMPTY; return TALSH_OBJECT_IS_EMPTY; }
+  if (!k->finalized) {
+    const hipError_t e = hipEventSynchronize(k->ev_finish);
+    task_finalize(t, e == hipSuccess ? TALSH_TASK_COMPLETED : TALSH_TASK_ERROR);
+  }
+  if (stats) *stats = k->status;
+  return 0;
+}
+
+int talshTasksWait(int ntasks, talsh_task_t tasks[], int stats[])
+{
+  if (ntasks <= 0 || !tasks) return TALSH_INVALID_ARGS;
+  int err = 0;
+  for (int i = 0; i < ntasks; ++i) {
+    int st = TALSH_TASK_EMPTY;
+    const int e = talshTaskWait(&tasks[i], &st);
+    if (stats) stats[i] = st;
+    if (e && !err) err = e;
+  }
+  return err;
+}
+
+int talshTaskTime(talsh_task_t *t, double *total, double *comput, double *input, double *output, double *mmul)
+{
+  if (!t || !total) return TALSH_INVALID_ARGS;
+  Task *k = task_of(t);
+  if (!k) return TALSH_OBJECT_IS_EMPTY;
+  if (talshTaskStatus(t) != TALSH_TASK_COMPLETED) return TALSH_IN_PROGRESS;
+  auto span = [](hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    return hipEventElapsedTime(&ms, a, b) == hipSuccess ? ms * 1e-3 : -1.0;
+  };
+  *total = span(k->ev_start, k->ev_finish);
+  if (input) *input = span(k->ev_start, k->ev_comput);
+  if (comput) *comput = span(k->ev_comput, k->ev_output);
+  if (output) *output = span(k->ev_output, k->ev_finish);
+  if (mmul) *mmul = k->has_mm ? span(k->ev_mm0, k->ev_mm1) : -1.0;
+  return 0;
+}
+int talshTaskTime_(talsh_task_t *t, double *total, double *comput, double *input, double *output, double *mmul)
+{
+  return talshTaskTime(t, total, comput, input, output, mmul);
+}
+
+void talshTaskPrint(const talsh_task_t *t)
+{
+  if (!t) return;
+  std::printf("#MESSAGE: TAL-SH task: error %d, device kind %d, data kind %d, coherence %d, args %d, flops %.6e\n",
+              t->task_error, t->dev_kind, t->data_kind, t->coherence, t->num_args, t->flops);
+}
+
+// ============================================================================ place / discard
+int talshTensorPlace(talsh_tens_t *tens, int dev_id, int dev_kind, void *dev_mem, int copy_ctrl, talsh_task_t *task)
+{
+  Engine &E = engine();
+  if (!E.on) return TALSH_NOT_INITIALIZED;
+  if (!tens || talshTensorIsEmpty(tens) == YEP) return TALSH_INVALID_ARGS;
+  const int flat = dev_kind == DEV_NULL ? canonical_dev(dev_id) : resolve_device(dev_id, dev_kind);
+  if (flat < 0) return TALSH_INVALID_ARGS;
+  if (find_image(tens, flat, NO_TYPE) >= 0) {
+    if (copy_ctrl == COPY_M)
+      for (int i = tens->ndev - 1; i >= 0; --i)
+        if (tens->dev_rsc[i].dev_id != flat) tensor_remove_image(tens, i, nullptr);
+    return 0;
+  }
+  if (tens->ndev <= 0) return TALSH_OBJECT_IS_EMPTY;
+  if (flat == 0) {
+    // device -> host: synchronous copy into a new host image
+    int src = -1;
+    for (int i = 0; i < tens->ndev && src < 0; ++i)
+      if (tens->avail[i] == YEP) src = i;
+    if (src < 0) return TALSH_IN_PROGRESS;
+    const int kind = tens->data_kind[src];
+    const size_t bytes = tensor_volume(tens) * (size_t)kind_size(kind);
+    talsh_dev_rsc_t r;
+    if (dev_mem) { r.dev_id = 0; r.gmem_p = dev_mem; r.buf_entry = -1; r.mem_attached = 1; }
+    else {
+      const int e = image_alloc(0, bytes, &r);
+      if (e) return e;
+    }
+    Device *d = gpu_by_flat(tens->dev_rsc[src].dev_id);
+    if (d) {
+      (void)hipSetDevice(d->ordinal);
+      for (auto s : d->streams) (void)hipStreamSynchronize(s);
+      if (!XT_HIP_OK(hipMemcpy(r.gmem_p, tens->dev_rsc[src].gmem_p, bytes, hipMemcpyDeviceToHost))) {
+        image_release(r, nullptr);
+        return TALSH_FAILURE;
+      }
+      d->stats.traffic_out += (double)bytes;
+    }
+    void *srcp = tens->dev_rsc[src].gmem_p;
+    tensor_add_image(tens, r, kind);
+    if (copy_ctrl == COPY_M || copy_ctrl == COPY_D) remove_by_ptr(tens, srcp);
+    if (task) {  // host placement completes synchronously
+      talshTaskClean(task);
+      task->task_error = 0;
+    }
+    return 0;
+  }
+  int err = 0;
+  Device *d = exec_device(flat, DEV_NULL, &err);
+  if (!d) return err;
+  std::vector<Arg> args(1);
+  args[0].t = tens;
+  args[0].need_input = true;
+  args[0].attach = dev_mem;
+  struct Body {
+    int allocate(Temps &) { return 0; }
+    int enqueue(Task *, Device *) { return 0; }
+  } body;
+  const int kind = tensor_kind(tens);
+  return run_task(task, d, (unsigned)(copy_ctrl & 3), args, kind, 0.0, body);
+}
+int talshTensorPlace_(talsh_tens_t *tens, int dev_id, int dev_kind, void *dev_mem, int copy_ctrl, talsh_task_t *task)
+{
+  return talshTensorPlace(tens, dev_id, dev_kind, dev_mem, copy_ctrl, task);
+}
+
+int talshTensorDiscard(talsh_tens_t *tens, int dev_id, int dev_kind)
+{
+  if (!engine().on) return TALSH_NOT_INITIALIZED;
+  if (!tens || talshTensorIsEmpty(tens) == YEP) return TALSH_INVALID_ARGS;
+  const int flat = dev_kind == DEV_NULL ? canonical_dev(dev_id) : resolve_device(dev_id, dev_kind);
+  if (flat < 0) return TALSH_INVALID_ARGS;
+  for (int i = tens->ndev - 1; i >= 0; --i) {
+    if (tens->dev_rsc[i].dev_id != flat) continue;
+    if (tens->avail[i] != YEP) return TALSH_IN_PROGRESS;
+    if (Device *d = gpu_by_flat(flat)) {
+      (void)hipSetDevice(d->ordinal);
+      for (auto s : d->streams) (void)hipStreamSynchronize(s);
+    }
+    tensor_remove_image(tens, i, nullptr);
+  }
+  return 0;
+}
+int talshTensorDiscard_(talsh_tens_t *tens, int dev_id, int dev_kind) { return talshTensorDiscard(tens, dev_id, dev_kind); }
+
+int talshTensorDiscardOther(talsh_tens_t *tens, int dev_id, int dev_kind)
+{
+  if (!engine().on) return TALSH_NOT_INITIALIZED;
+  if (!tens || talshTensorIsEmpty(tens) == YEP) return TALSH_INVALID_ARGS;
+  const int flat = dev_kind == DEV_NULL ? canonical_dev(dev_id) : resolve_device(dev_id, dev_kind);
+  if (flat < 0) return TALSH_INVALID_ARGS;
+  if (find_image(tens, flat, NO_TYPE) < 0) return TALSH_NOT_FOUND;
+  for (int i = tens->ndev - 1; i >= 0; --i) {
+    if (tens->dev_rsc[i].dev_id == flat) continue;
+    if (tens->avail[i] != YEP) return TALSH_IN_PROGRESS;
+    tensor_remove_image(tens, i, nullptr);
+  }
+  return 0;
+}
+int talshTensorDiscardOther_(talsh_tens_t *tens, int dev_id, int dev_kind)
+{
+  return talshTensorDiscardOther(tens, dev_id, dev_kind);
+}
+
+// ============================================================================ init / scale
+int talshTensorInit(talsh_tens_t *dtens, double vr, double vi, int dev_id, int dev_kind, int copy_ctrl,
+                    talsh_task_t *task)
+{
+  int err = 0;
+  Device *d = exec_device(dev_id, dev_kind, &err);
+  if (!d) return err;
+  if (!dtens || talshTensorIsEmpty(dtens) == YEP) return TALSH_INVALID_ARGS;
+  std::vector<Arg> args(1);
+  args[0].t = dtens;
+  args[0].output = true;
+  args[0].need_input = false;
+  struct Body {
+    Arg *a; double re, im;
+    int allocate(Temps &) { return 0; }
+    int enqueue(Task *k, Device *) { return launch_fill(a->exec, a->kind, tensor_volume(a->t), re, im, k->stream); }
+  } body{&args[0], vr, vi};
+  return run_task(task, d, (unsigned)copy_ctrl, args, tensor_kind(dtens), 0.0, body);
+}
+int talshTensorInit_(talsh_tens_t *dtens, double vr, double vi, int dev_id, int dev_kind, int copy_ctrl,
+                     talsh_task_t *task)
+{
+  return talshTensorInit(dtens, vr, vi, dev_id, dev_kind, copy_ctrl, task);
+}
+
+int talshTensorScale(talsh_tens_t *dtens, double vr, double vi, int dev_id, int dev_kind, int copy_ctrl,
+                     talsh_task_t *task)
+{
+  int err = 0;
+  Device *d = exec_device(dev_id, dev_kind, &err);
+  if (!d) return err;
+  if (!dtens || talshTensorIsEmpty(dtens) == YEP) return TALSH_INVALID_ARGS;
+  std::vector<Arg> args(1);
+  args[0].t = dtens;
+  args[0].output = true;
+  struct Body {
+    Arg *a; double re, im;
+    int allocate(Temps &) { return 0; }
+    int enqueue(Task *k, Device *) { return launch_scale(a->exec, a->kind, tensor_volume(a->t), re, im, k->stream); }
+  } body{&args[0], vr, vi};
+  return run_task(task, d, (unsigned)copy_ctrl, args, tensor_kind(dtens), (double)tensor_volume(dtens), body);
+}
+int talshTensorScale_(talsh_tens_t *dtens, double vr, double vi, int dev_id, int dev_kind, int copy_ctrl,
+                      talsh_task_t *task)
+{
+  return talshTensorScale(dtens, vr, vi, dev_id, dev_kind, copy_ctrl, task);
+}
+
+// ============================================================================ slice / insert (not on this engine)
+int talshTensorSlice(talsh_tens_t *, talsh_tens_t *, const int *, int, int, int, int, talsh_task_t *)
+{
+  return TALSH_NOT_IMPLEMENTED;
+}
+int talshTensorSlice_(talsh_tens_t *, talsh_tens_t *, const int *, int, int, int, int, talsh_task_t *)
+{
+  return TALSH_NOT_IMPLEMENTED;
+}
+int talshTensorInsert(talsh_tens_t *, talsh_tens_t *, const int *, int, int, int, int, talsh_task_t *)
+{
+  return TALSH_NOT_IMPLEMENTED;
+}
+int talshTensorInsert_(talsh_tens_t *, talsh_tens_t *, const int *, int, int, int, int, talsh_task_t *)
+{
+  return TALSH_NOT_IMPLEMENTED;
+}
+
+// ============================================================================ copy / add
+int talshTensorCopy(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, int dev_id, int dev_kind,
+                    int copy_ctrl, talsh_task_t *task)
+{
+  int err = 0;
+  Device *d = exec_device(dev_id, dev_kind, &err);
+  if (!d) return err;
+  if (!cptrn || !dtens || !ltens) return TALSH_INVALID_ARGS;
+  if (talshTensorIsEmpty(dtens) == YEP || talshTensorIsEmpty(ltens) == YEP) return TALSH_OBJECT_IS_EMPTY;
+  int dig[2 * MAX_TENSOR_RANK], dr, lr, rr, cj;
+  if (talsh_get_contr_ptrn_str2dig(cptrn, dig, &dr, &lr, &rr, &cj)) return TALSH_INVALID_ARGS;
+  if (rr != 0 || dr != lr || dr != talshTensorRank(dtens) || lr != talshTensorRank(ltens)) return TALSH_INVALID_ARGS;
+  const std::vector<int> ld = dims_of(ltens), dd = dims_of(dtens);
+  std::vector<int> o2n(lr);
+  for (int i = 0; i < lr; ++i) {
+    o2n[i] = dig[i] - 1;
+    if (o2n[i] < 0 || o2n[i] >= dr || dd[o2n[i]] != ld[i]) return TALSH_INVALID_ARGS;
+  }
+  bool lconj = cj & 2;
+  if (cj & 1) lconj = !lconj;
+  std::vector<Arg> args(2);
+  args[0].t = dtens; args[0].output = true; args[0].need_input = false;
+  args[1].t = ltens;
+  struct Body {
+    Arg *D, *L; std::vector<int> ld, o2n; bool conj;
+    int allocate(Temps &) { return 0; }
+    int enqueue(Task *k, Device *)
+    {
+      hipEvent_t pr[2];
+      prof_begin(k->stream, kKTranspose, pr);
+      const int e = launch_permute(L->exec, D->exec, L->kind, (int)ld.size(), ld.data(), o2n.data(), conj, k->stream);
+      prof_end(k->stream, kKTranspose, pr);
+      return e;
+    }
+  } body{&args[0], &args[1], ld, o2n, lconj};
+  const int kind = tensor_kind(dtens);
+  return run_task(task, d, (unsigned)copy_ctrl, args, kind, 0.0, body);
+}
+int talshTensorCopy_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, int dev_id, int dev_kind,
+                     int copy_ctrl, talsh_task_t *task)
+{
+  return talshTensorCopy(cptrn, dtens, ltens, dev_id, dev_kind, copy_ctrl, task);
+}
+
+int talshTensorAdd(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, double sr, double si, int dev_id,
+                   int dev_kind, int copy_ctrl, talsh_task_t *task)
+{
+  int err = 0;
+  Device *d = exec_device(dev_id, dev_kind, &err);
+  if (!d) return err;
+  if (!cptrn || !dtens || !ltens) return TALSH_INVALID_ARGS;
+  if (talshTensorIsEmpty(dtens) == YEP || talshTensorIsEmpty(ltens) == YEP) return TALSH_OBJECT_IS_EMPTY;
+  int dig[2 * MAX_TENSOR_RANK], dr, lr, rr, cj;
+  if (talsh_get_contr_ptrn_str2dig(cptrn, dig, &dr, &lr, &rr, &cj)) return TALSH_INVALID_ARGS;
+  if (rr != 0 || dr != lr || dr != talshTensorRank(dtens) || lr != talshTensorRank(ltens)) return TALSH_INVALID_ARGS;
+  const std::vector<int> ld = dims_of(ltens), dd = dims_of(dtens);
+  std::vector<int> o2n(lr);
+  for (int i = 0; i < lr; ++i) {
+    o2n[i] = dig[i] - 1;
+    if (o2n[i] < 0 || o2n[i] >= dr || dd[o2n[i]] != ld[i]) return TALSH_INVALID_ARGS;
+  }
+  bool lconj = cj & 2;
+  if (cj & 1) lconj = !lconj;
+  std::vector<Arg> args(2);
+  args[0].t = dtens; args[0].output = true;
+  args[1].t = ltens;
+  struct Body {
+    Arg *D, *L; std::vector<int> ld, o2n; bool conj; double sr, si; void *tmp = nullptr;
+    int allocate(Temps &t)
+    {
+      if (is_identity(o2n)) return 0;
+      return t.alloc(tensor_volume(L->t) * (size_t)kind_size(tensor_kind(L->t)), &tmp);
+    }
+    int enqueue(Task *k, Device *)
+    {
+      const void *src = L->exec;
+      if (tmp) {
+        int e = launch_permute(L->exec, tmp, L->kind, (int)ld.size(), ld.data(), o2n.data(), false, k->stream);
+        if (e) return e;
+        src = tmp;
+      }
+      return launch_axpy(D->exec, src, D->kind, tensor_volume(D->t), sr, si, conj && kind_complex(D->kind), true,
+                         k->stream);
+    }
+  } body{&args[0], &args[1], ld, o2n, lconj, sr, si};
+  return run_task(task, d, (unsigned)copy_ctrl, args, tensor_kind(dtens), 2.0 * (double)tensor_volume(dtens), body);
+}
+int talshTensorAdd_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, double sr, double si, int dev_id,
+                    int dev_kind, int copy_ctrl, talsh_task_t *task)
+{
+  return talshTensorAdd(cptrn, dtens, ltens, sr, si, dev_id, dev_kind, copy_ctrl, task);
+}
+
+// ============================================================================ contract
+int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens, double sr,
+                        double si, int dev_id, int dev_kind, int copy_ctrl, int accumulative, talsh_task_t *task)
+{
+  int err = 0;
+  Device *d = exec_device(dev_id, dev_kind, &err);
+  if (!d) return err;
+  if (!cptrn || !dtens || !ltens || !rtens) return TALSH_INVALID_ARGS;
+  if (talshTensorIsEmpty(dtens) == YEP || talshTensorIsEmpty(ltens) == YEP || talshTensorIsEmpty(rtens) == YEP)
+    return TALSH_OBJECT_IS_EMPTY;
+  int digc[2 * MAX_TENSOR_RANK], dr, lr, rr, cj;
+  if (talsh_get_contr_ptrn_str2dig(cptrn, digc, &dr, &lr, &rr, &cj)) return TALSH_INVALID_ARGS;
+  if (rr < 0 || dr != talshTensorRank(dtens) || lr != talshTensorRank(ltens) || rr != talshTensorRank(rtens))
+    return TALSH_INVALID_ARGS;
+  std::vector<int> dig(digc, digc + lr + rr);
+  const std::vector<int> dd = dims_of(dtens), ld = dims_of(ltens), rd = dims_of(rtens);
+  // extents must agree along every paired index
+  for (int i = 0; i < lr; ++i) {
+    const int v = dig[i];
+    if (v > 0 ? (v > dr || dd[v - 1] != ld[i]) : (-v > rr || rd[-v - 1] != ld[i])) return TALSH_INVALID_ARGS;
+  }
+  for (int j = 0; j < rr; ++j) {
+    const int v = dig[lr + j];
+    if (v > 0 ? (v > dr || dd[v - 1] != rd[j]) : (-v > lr || ld[-v - 1] != rd[j])) return TALSH_INVALID_ARGS;
+  }
+  {  // every D index must come from L or R
+    std::vector<int> cnt(dr + 1, 0);
+    for (int v : dig)
+      if (v > 0) cnt[v]++;
+    for (int q = 1; q <= dr; ++q)
+      if (cnt[q] == 0) return TALSH_INVALID_ARGS;
+  }
+  const int kind = tensor_kind(dtens);
+  const bool cplx = kind_complex(kind);
+  bool lconj = cplx && (cj & 2), rconj = cplx && (cj & 4);
+  if (cplx && (cj & 1)) { lconj = !lconj; rconj = !rconj; }
+  const bool acc = accumulative != NOPE;
+  const int es = kind_size(kind);
+  std::vector<Arg> args(3);
+  args[0].t = dtens; args[0].output = true; args[0].need_input = acc;
+  args[1].t = ltens;
+  args[2].t = rtens;
+  ContractPlan P;
+  if (dr == 0 && lr == 0 && rr == 0) P.kase = 0;
+  else if ((lr == 0) != (rr == 0)) P.kase = 1;
+  else if (dr == 0) P.kase = 2;
+  else {
+    const int e = plan_contraction(dig, dr, lr, rr, dd, ld, rd, es, acc, g_plan_variant, P, nullptr);
+    if (e) return e;
+  }
+  size_t ll = 1, lrr = 1, lc = 1;
+  for (int i = 0; i < lr; ++i) (dig[i] > 0 ? ll : lc) *= (size_t)ld[i];
+  for (int j = 0; j < rr; ++j)
+    if (dig[lr + j] > 0) lrr *= (size_t)rd[j];
+  const double flops = (cplx ? 8.0 : 2.0) * (double)ll * (double)lrr * (double)lc;
+  struct Body {
+    Arg *D, *L, *R; ContractPlan P; std::vector<int> dig, dd, ld, rd; int lr, rr; bool lconj, rconj, acc;
+    double are, aim; int tile;
+    void *lt = nullptr, *rt = nullptr, *dt = nullptr, *scr = nullptr, *tt = nullptr;
+    int allocate(Temps &t)
+    {
+      const size_t es = kind_size(D->kind);
+      if (P.kase == 1) {
+        const bool l_is_tensor = lr > 0;
+        const std::vector<int> &td = l_is_tensor ? ld : rd;
+        std::vector<int> o2n(td.size());
+        for (size_t i = 0; i < td.size(); ++i) o2n[i] = dig[(l_is_tensor ? 0 : lr) + i] - 1;
+        if (!is_identity(o2n)) return t.alloc(tensor_volume(D->t) * es, &tt);
+        return 0;
+      }
+      if (P.kase == 0) return t.alloc(1024 * sizeof(double2), &scr);
+      if (P.kase == 2) {
+        int e = t.alloc(1024 * sizeof(double2), &scr);
+        if (e) return e;
+        // R reordered into L's contracted order when they differ
+        std::vector<int> o2n(rr);
+        for (int j = 0; j < rr; ++j) o2n[j] = -dig[lr + j] - 1;
+        if (!is_identity(o2n)) return t.alloc(tensor_volume(R->t) * es, &rt);
+        return 0;
+      }
+      int e = 0;
+      if (!P.l_o2n.empty()) e = t.alloc(tensor_volume(L->t) * es, &lt);
+      if (!e && !P.r_o2n.empty()) e = t.alloc(tensor_volume(R->t) * es, &rt);
+      if (!e && !P.d_in_o2n.empty()) e = t.alloc(tensor_volume(D->t) * es, &dt);
+      return e;
+    }
+    int enqueue(Task *k, Device *dev)
+    {
+      hipStream_t s = k->stream;
+      const int kind = D->kind;
+      hipEvent_t pr[2];
+      if (P.kase == 0)
+        return launch_dot(D->exec, L->exec, R->exec, kind, 1, are, aim, lconj, rconj, acc, scr,
+                          1024 * sizeof(double2), s);
+      if (P.kase == 1) {
+        const bool l_is_tensor = lr > 0;
+        Arg *T = l_is_tensor ? L : R, *S = l_is_tensor ? R : L;
+        const std::vector<int> &td = l_is_tensor ? ld : rd;
+        std::vector<int> o2n(td.size());
+        for (size_t i = 0; i < td.size(); ++i) o2n[i] = dig[(l_is_tensor ? 0 : lr) + i] - 1;
+        const void *src = T->exec;
+        if (tt) {
+          int e = launch_permute(T->exec, tt, kind, (int)td.size(), td.data(), o2n.data(), false, s);
+          if (e) return e;
+          src = tt;
+        }
+        return launch_axpy_dev(D->exec, src, S->exec, kind, tensor_volume(D->t), are, aim,
+                               l_is_tensor ? lconj : rconj, l_is_tensor ? rconj : lconj, acc, s);
+      }
+      if (P.kase == 2) {
+        const void *rsrc = R->exec;
+        if (rt) {
+          std::vector<int> o2n(rr);
+          for (int j = 0; j < rr; ++j) o2n[j] = -dig[lr + j] - 1;
+          int e = launch_permute(R->exec, rt, kind, rr, rd.data(), o2n.data(), false, s);
+          if (e) return e;
+          rsrc = rt;
+        }
+        return launch_dot(D->exec, L->exec, rsrc, kind, tensor_volume(L->t), are, aim, lconj, rconj, acc, scr,
+                          1024 * sizeof(double2), s);
+      }
+      // GEMM path
+      const void *la = L->exec, *rb = R->exec;
+      void *dc = D->exec;
+      if (lt) {
+        prof_begin(s, kKTranspose, pr);
+        int e = launch_permute(L->exec, lt, kind, lr, ld.data(), P.l_o2n.data(), false, s);
+        prof_end(s, kKTranspose, pr);
+        if (e) return e;
+        la = lt;
+      }
+      if (rt) {
+        prof_begin(s, kKTranspose, pr);
+        int e = launch_permute(R->exec, rt, kind, rr, rd.data(), P.r_o2n.data(), false, s);
+        prof_end(s, kKTranspose, pr);
+        if (e) return e;
+        rb = rt;
+      }
+      if (dt) {
+        if (acc) {
+          prof_begin(s, kKTranspose, pr);
+          int e = launch_permute(D->exec, dt, kind, (int)dd.size(), dd.data(), P.d_in_o2n.data(), false, s);
+          prof_end(s, kKTranspose, pr);
+          if (e) return e;
+        }
+        dc = dt;
+      }
+      GemmArgs g{};
+      g.kind = kind;
+      g.r4mode = dev->r4mode;
+      g.tile = tile;
+      g.beta_zero = !acc;
+      g.alpha_re = are;
+      g.alpha_im = kind_complex(kind) ? aim : 0.0;
+      if (!P.swap) {
+        g.m = P.m; g.n = P.n; g.k = P.k; g.a = la; g.b = rb; g.conj_a = lconj; g.conj_b = rconj;
+      } else {
+        g.m = P.n; g.n = P.m; g.k = P.k; g.a = rb; g.b = la; g.conj_a = rconj; g.conj_b = lconj;
+      }
+      g.a_kfast = P.a_kfast;
+      g.b_kfast = P.b_kfast;
+      g.c = dc;
+      (void)hipEventRecord(k->ev_mm0, s);
+      prof_begin(s, kKGemm, pr);
+      int e = launch_gemm(g, s);
+      prof_end(s, kKGemm, pr);
+      (void)hipEventRecord(k->ev_mm1, s);
+      k->has_mm = true;
+      if (e) return e;
+      if (dt) {
+        prof_begin(s, kKTranspose, pr);
+        e = launch_permute(dt, D->exec, kind, (int)P.dmat_dims.size(), P.dmat_dims.data(), P.d_out_o2n.data(), false,
+                           s);
+        prof_end(s, kKTranspose, pr);
+      }
+      return e;
+    }
+  } body{&args[0], &args[1], &args[2], P, dig, dd, ld, rd, lr, rr, lconj, rconj, acc, sr, cplx ? si : 0.0,
+         g_gemm_tile};
+  if (!cplx) body.aim = si;  // real kinds: GEMM paths use Re(alpha); the scaled-add case needs the full factor
+  if (!cplx && P.kase != 1) body.aim = 0.0;
+  return run_task(task, d, (unsigned)copy_ctrl, args, kind, flops, body);
+}
+
+int talshTensorContract_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens, double sr,
+                         double si, int dev_id, int dev_kind, int copy_ctrl, int accumulative, talsh_task_t *task)
+{
+  return talshTensorContract(cptrn, dtens, ltens, rtens, sr, si, dev_id, dev_kind, copy_ctrl, accumulative, task);
+}
+
+int talshTensorContractXL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens, double sr,
+                          double si, int dev_id, int dev_kind, int accumulative)
+{
+  // Operands that fit the device arena run as one blocking contraction;
+  // out-of-memory decomposition is not provided by this engine.
+  return talshTensorContract(cptrn, dtens, ltens, rtens, sr, si, dev_id, dev_kind, COPY_MTT, accumulative, nullptr);
+}
+int talshTensorContractXL_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                           double sr, double si, int dev_id, int dev_kind, int accumulative)
+{
+  return talshTensorContractXL(cptrn, dtens, ltens, rtens, sr, si, dev_id, dev_kind, accumulative);
+}
+
+// ============================================================================ not provided by this engine
+int talshTensorDecomposeSVD(const char *, talsh_tens_t *, talsh_tens_t *, talsh_tens_t *, talsh_tens_t *, const char,
+                            int, int)
+{
+  return TALSH_NOT_IMPLEMENTED;
+}
+int talshTensorDecomposeSVDL(const char *, talsh_tens_t *, talsh_tens_t *, talsh_tens_t *, int, int)
+{
+  return TALSH_NOT_IMPLEMENTED;
+}
+int talshTensorDecomposeSVDR(const char *, talsh_tens_t *, talsh_tens_t *, talsh_tens_t *, int, int)
+{
+  return TALSH_NOT_IMPLEMENTED;
+}
+int talshTensorDecomposeSVDLR(const char *, talsh_tens_t *, talsh_tens_t *, talsh_tens_t *, int, int)
+{
+  return TALSH_NOT_IMPLEMENTED;
+}
+int talshTensorOrthogonalizeSVD(const char *, talsh_tens_t *, int, int) { return TALSH_NOT_IMPLEMENTED; }
+int talshTensorOrthogonalizeMGS(talsh_tens_t *, int, int *, int, int) { return TALSH_NOT_IMPLEMENTED; }
+
+}  // extern "C"
+
+// ============================================================================ engine extensions
+extern "C" {
+
+int talshAmdProfile(int on)
+{
+  engine().profiling = on ? 1 : 0;
+  return 0;
+}
+
+int talshAmdProfileQuery(int kernel_class, double *total_ms, long long *launches)
+{
+  if (!total_ms || !launches) return TALSH_INVALID_ARGS;
+  return prof_collect(kernel_class, total_ms, launches);
+}
+
+int talshAmdPlanCandidates(const char *cptrn, int drank, const int *ddims, int lrank, const int *ldims, int rrank,
+                           const int *rdims, int data_kind, int accumulative, int cap, double *moved, int *ncand)
+{
+  if (!cptrn || !ncand) return TALSH_INVALID_ARGS;
+  int digc[2 * MAX_TENSOR_RANK], dr, lr, rr, cj;
+  if (talsh_get_contr_ptrn_str2dig(cptrn, digc, &dr, &lr, &rr, &cj)) return TALSH_INVALID_ARGS;
+  if (dr != drank || lr != lrank || rr != rrank || dr == 0 || lr == 0 || rr == 0) return TALSH_INVALID_ARGS;
+  std::vector<int> dig(digc, digc + lr + rr), dd(ddims, ddims + dr), ld(ldims, ldims + lr), rd(rdims, rdims + rr);
+  ContractPlan P;
+  std::vector<ContractPlan> all;
+  const int e = plan_contraction(dig, dr, lr, rr, dd, ld, rd, kind_size(data_kind), accumulative != NOPE, -1, P, &all);
+  if (e) return e;
+  *ncand = (int)all.size();
+  for (int i = 0; i < (int)all.size() && i < cap; ++i)
+    if (moved) moved[i] = all[i].bytes_moved;
+  return 0;
+}
+
+int talshAmdDeviceSynchronize(int dev_num)
+{
+  Engine &E = engine();
+  if (dev_num < 0 || dev_num >= (int)E.gpus.size()) return TALSH_INVALID_ARGS;
+  Device *d = E.gpus[dev_num];
+  (void)hipSetDevice(d->ordinal);
+  for (auto s : d->streams)
+    if (!XT_HIP_OK(hipStreamSynchronize(s))) return TALSH_FAILURE;
+  return 0;
+}
+
+}  // extern "C"

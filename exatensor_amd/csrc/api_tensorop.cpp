@@ -1,0 +1,181 @@
+// talsh.h tensor-operation objects (TALSH/talshc.cpp:2598-3300).
+//
+// The object lifecycle, argument/specification setters and the flop/byte
+// accounting are provided. The staged execution used by the out-of-memory
+// driver (activate / load input / execute / store output / decompose) is not
+// part of this engine yet and reports TALSH_NOT_IMPLEMENTED.
+#include <cstring>
+
+#include "common.hpp"
+
+extern "C" {
+
+int talshTensorOpCreate(talsh_tens_op_t **op)
+{
+  if (!op) return TALSH_INVALID_ARGS;
+  *op = (talsh_tens_op_t *)std::malloc(sizeof(talsh_tens_op_t));
+  if (!*op) return TRY_LATER;
+  return talshTensorOpClean(*op);
+}
+
+int talshTensorOpClean(talsh_tens_op_t *op)
+{
+  if (!op) return TALSH_INVALID_ARGS;
+  op->opkind = TALSH_TENSOR_NOOP;
+  op->data_kind = NO_TYPE;
+  op->num_args = 0;
+  for (int i = 0; i < MAX_TENSOR_OPERANDS; ++i) {
+    talshTensorSliceClean(&op->tens_slice[i]);
+    talshTensorClean(&op->tens_arg[i]);
+  }
+  op->symb_pattern = nullptr;
+  op->alpha_real = 1.0;
+  op->alpha_imag = 0.0;
+  talshTaskClean(&op->task_handle);
+  op->exec_dev_id = DEV_NULL;
+  op->stage = TALSH_OP_EMPTY;
+  op->time_started = op->time_scheduled = op->time_completed = op->time_finished = -1.0;
+  return 0;
+}
+
+int talshTensorOpSetArgument(talsh_tens_op_t *op, const talsh_tens_t *tensor, const size_t *offsets, const int *dims)
+{
+  if (!op || !tensor) return TALSH_INVALID_ARGS;
+  if (op->num_args >= MAX_TENSOR_OPERANDS) return TALSH_LIMIT_EXCEEDED;
+  const int e = talshTensorSliceConstruct(&op->tens_slice[op->num_args], tensor, offsets, dims, nullptr, nullptr);
+  if (e) return e;
+  ++op->num_args;
+  op->stage = TALSH_OP_PARTIAL;
+  return 0;
+}
+
+int talshTensorOpSpecify(talsh_tens_op_t *op, int operation_kind, int data_kind, const char *symbolic_pattern,
+                         double prefactor_real, double prefactor_imag)
+{
+  if (!op) return TALSH_INVALID_ARGS;
+  if (tens_valid_data_kind(data_kind, nullptr) != YEP) return TALSH_INVALID_ARGS;
+  op->opkind = operation_kind;
+  op->data_kind = data_kind;
+  op->symb_pattern = symbolic_pattern;
+  op->alpha_real = prefactor_real;
+  op->alpha_imag = prefactor_imag;
+  op->stage = TALSH_OP_DEFINED;
+  return 0;
+}
+
+int talshTensorOpSetExecDevice(talsh_tens_op_t *op, int dev_id, int dev_kind)
+{
+  if (!op) return TALSH_INVALID_ARGS;
+  op->exec_dev_id = dev_kind == DEV_DEFAULT ? dev_id : talshFlatDevId(dev_kind, dev_id);
+  return 0;
+}
+
+int talshTensorOpActivate(talsh_tens_op_t *) { return TALSH_NOT_IMPLEMENTED; }
+int talshTensorOpLoadInput(talsh_tens_op_t *) { return TALSH_NOT_IMPLEMENTED; }
+int talshTensorOpExecute(talsh_tens_op_t *, int, int) { return TALSH_NOT_IMPLEMENTED; }
+int talshTensorOpTest(talsh_tens_op_t *, int *completed, int)
+{
+  if (completed) *completed = NOPE;
+  return TALSH_NOT_IMPLEMENTED;
+}
+int talshTensorOpStoreOutput(talsh_tens_op_t *) { return TALSH_NOT_IMPLEMENTED; }
+int talshTensorOpDeactivate(talsh_tens_op_t *) { return TALSH_NOT_IMPLEMENTED; }
+int talshTensorOpProgress(talsh_tens_op_t *, int *done)
+{
+  if (done) *done = NOPE;
+  return TALSH_NOT_IMPLEMENTED;
+}
+int talshTensorOpDecompose2(const talsh_tens_op_t *, talsh_tens_op_t *, talsh_tens_op_t *)
+{
+  return TALSH_NOT_IMPLEMENTED;
+}
+
+int talshTensorOpDestruct(talsh_tens_op_t *op)
+{
+  if (!op) return TALSH_INVALID_ARGS;
+  if (talshTaskIsEmpty(&op->task_handle) != YEP) {
+    const int e = talshTaskDestruct(&op->task_handle);
+    if (e) return e;
+  }
+  for (int i = 0; i < MAX_TENSOR_OPERANDS; ++i) {
+    talshTensorSliceDestruct(&op->tens_slice[i]);
+    if (talshTensorIsEmpty(&op->tens_arg[i]) != YEP) talshTensorDestruct(&op->tens_arg[i]);
+  }
+  return talshTensorOpClean(op);
+}
+
+int talshTensorOpDestroy(talsh_tens_op_t *op)
+{
+  if (!op) return TALSH_INVALID_ARGS;
+  const int e = talshTensorOpDestruct(op);
+  std::free(op);
+  return e;
+}
+
+size_t talshTensorOpGetArgVolume(const talsh_tens_op_t *op, unsigned int arg_num)
+{
+  if (!op || arg_num >= op->num_args) return 0;
+  return talshTensorSliceVolume(&op->tens_slice[arg_num]);
+}
+
+size_t talshTensorOpGetArgSize(const talsh_tens_op_t *op, unsigned int arg_num)
+{
+  int es = 0;
+  if (!op || tens_valid_data_kind(op->data_kind, &es) != YEP) return 0;
+  return talshTensorOpGetArgVolume(op, arg_num) * (size_t)es;
+}
+
+double talshTensorOpGetByteCount(const talsh_tens_op_t *op, unsigned int element_size)
+{
+  if (!op || op->opkind == TALSH_TENSOR_NOOP) return 0.0;
+  int es = (int)element_size;
+  if (op->data_kind != NO_TYPE && tens_valid_data_kind(op->data_kind, &es) != YEP) return 0.0;
+  double b = 0;
+  for (unsigned i = 0; i < op->num_args; ++i) b += (double)talshTensorSliceVolume(&op->tens_slice[i]) * es;
+  return b;
+}
+
+double talshTensorOpGetFlopCount(const talsh_tens_op_t *op)
+{
+  // flops = fma * (product of distinct index extents), TALSH/talshc.cpp:3090-3139
+  if (!op) return 0.0;
+  double fma;
+  if (op->data_kind == R4 || op->data_kind == R8) fma = 2.0;
+  else if (op->data_kind == C4 || op->data_kind == C8) fma = 8.0;
+  else return 0.0;
+  if (op->opkind != TALSH_TENSOR_CONTRACT || !op->symb_pattern || op->num_args < 3) return 0.0;
+  int dig[2 * MAX_TENSOR_RANK], dr, lr, rr, cj;
+  if (talsh_get_contr_ptrn_str2dig(op->symb_pattern, dig, &dr, &lr, &rr, &cj)) return 0.0;
+  std::vector<int> seen(dr + 1, 0);
+  double f = 1.0;
+  for (int i = 0; i < lr; ++i) {
+    const double nd = op->tens_slice[1].shape.dims[i];
+    if (dig[i] > 0) {
+      if (!seen[dig[i]]) { seen[dig[i]] = 1; f *= nd; }
+    } else {
+      f *= nd;
+    }
+  }
+  for (int j = 0; j < rr; ++j) {
+    const double nd = op->tens_slice[2].shape.dims[j];
+    const int v = dig[lr + j];
+    if (v > 0 && !seen[v]) { seen[v] = 1; f *= nd; }
+  }
+  return f * fma;
+}
+
+double talshTensorOpGetIntensity(const talsh_tens_op_t *op)
+{
+  const double f = talshTensorOpGetFlopCount(op), b = talshTensorOpGetByteCount(op, 1);
+  if (b <= 0.0 || f < 0.0) return -1.0;
+  return f / b;
+}
+
+void talshTensorOpPrint(const talsh_tens_op_t *op)
+{
+  if (!op) return;
+  std::printf("#MESSAGE: Tensor operation: kind %d, data kind %d, args %u, stage %d, pattern %s\n", op->opkind,
+              op->data_kind, op->num_args, op->stage, op->symb_pattern ? op->symb_pattern : "(none)");
+}
+
+}  // extern "C"

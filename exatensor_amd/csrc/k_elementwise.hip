@@ -1,0 +1,318 @@
+// Element-wise tensor-body kernels for gfx950: fill (talshTensorInit), scale
+// (talshTensorScale), scaled accumulate (talshTensorAdd and the rank-0 operand
+// case of a contraction), and the full contraction to a scalar.
+//
+// Replace gpu_array_init__/gpu_array_scale__/gpu_array_add__/
+// gpu_array_dot_product__ (TALSH/tensor_algebra_gpu_nvidia.cu:337-683).
+// Arithmetic uses explicitly rounded products and sums (no fused multiply-add)
+// so results equal the CPU path's separate multiply and add bit for bit.
+#include <cmath>
+
+#include "kernels.hpp"
+
+// Products and sums below are rounded separately: no contraction into FMA.
+#pragma clang fp contract(off)
+
+namespace xt {
+namespace {
+
+constexpr int kNT = 256;
+
+__device__ __forceinline__ float mul(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ double mul(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ float add(float a, float b) { return __fadd_rn(a, b); }
+__device__ __forceinline__ double add(double a, double b) { return __dadd_rn(a, b); }
+__device__ __forceinline__ float sub(float a, float b) { return __fsub_rn(a, b); }
+__device__ __forceinline__ double sub(double a, double b) { return __dsub_rn(a, b); }
+
+template <typename R>
+struct Cx { R re, im; };
+
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_fill(T *__restrict__ d, uint64_t n, T v)
+{
+  const uint64_t st = (uint64_t)gridDim.x * kNT;
+  for (uint64_t i = (uint64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += st) d[i] = v;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kNT) void k_scale_real(R *__restrict__ d, uint64_t n, R f)
+{
+  const uint64_t st = (uint64_t)gridDim.x * kNT;
+  for (uint64_t i = (uint64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += st) d[i] = mul(d[i], f);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kNT) void k_scale_cplx(Cx<R> *__restrict__ d, uint64_t n, R fr, R fi)
+{
+  const uint64_t st = (uint64_t)gridDim.x * kNT;
+  for (uint64_t i = (uint64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += st) {
+    const Cx<R> x = d[i];
+    d[i] = Cx<R>{sub(mul(x.re, fr), mul(x.im, fi)), add(mul(x.re, fi), mul(x.im, fr))};
+  }
+}
+
+template <typename R, bool ACC>
+__global__ __launch_bounds__(kNT) void k_axpy_real(R *__restrict__ d, const R *__restrict__ l, uint64_t n, R f)
+{
+  const uint64_t st = (uint64_t)gridDim.x * kNT;
+  for (uint64_t i = (uint64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += st) {
+    const R t = mul(l[i], f);
+    d[i] = ACC ? add(d[i], t) : t;
+  }
+}
+
+template <typename R, bool ACC, bool CONJ>
+__global__ __launch_bounds__(kNT) void k_axpy_cplx(Cx<R> *__restrict__ d, const Cx<R> *__restrict__ l, uint64_t n,
+                                                    R fr, R fi)
+{
+  const uint64_t st = (uint64_t)gridDim.x * kNT;
+  for (uint64_t i = (uint64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += st) {
+    Cx<R> x = l[i];
+    if (CONJ) x.im = -x.im;
+    const R tr = sub(mul(x.re, fr), mul(x.im, fi)), ti = add(mul(x.re, fi), mul(x.im, fr));
+    if (ACC) {
+      const Cx<R> y = d[i];
+      d[i] = Cx<R>{add(y.re, tr), add(y.im, ti)};
+    } else {
+      d[i] = Cx<R>{tr, ti};
+    }
+  }
+}
+
+// Two-pass dot product: per-workgroup partial sums in double, then one
+// workgroup reduces them (deterministic order, no atomics).
+template <typename R, bool CPLX>
+__global__ __launch_bounds__(kNT) void k_dot_partial(const R *__restrict__ l, const R *__restrict__ r, uint64_t n,
+                                                      bool lconj, bool rconj, double2 *__restrict__ part)
+{
+  __shared__ double sr[kNT], si[kNT];
+  double ar = 0, ai = 0;
+  const uint64_t st = (uint64_t)gridDim.x * kNT;
+  for (uint64_t i = (uint64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += st) {
+    if (CPLX) {
+      const double xr = l[2 * i], yr = r[2 * i];
+      const double xi = lconj ? -(double)l[2 * i + 1] : (double)l[2 * i + 1];
+      const double yi = rconj ? -(double)r[2 * i + 1] : (double)r[2 * i + 1];
+      ar += xr * yr - xi * yi;
+      ai += xr * yi + xi * yr;
+    } else {
+      ar += (double)l[i] * (double)r[i];
+    }
+  }
+  sr[threadIdx.x] = ar;
+  si[threadIdx.x] = ai;
+  __syncthreads();
+  for (int w = kNT / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sr[threadIdx.x] += sr[threadIdx.x + w];
+      si[threadIdx.x] += si[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = make_double2(sr[0], si[0]);
+}
+
+template <typename R, bool CPLX>
+__global__ __launch_bounds__(kNT) void k_dot_final(const double2 *__restrict__ part, int np, R *__restrict__ d,
+                                                    double are, double aim, bool acc)
+{
+  __shared__ double sr[kNT], si[kNT];
+  double ar = 0, ai = 0;
+  for (int i = threadIdx.x; i < np; i += kNT) {
+    ar += part[i].x;
+    ai += part[i].y;
+  }
+  sr[threadIdx.x] = ar;
+  si[threadIdx.x] = ai;
+  __syncthreads();
+  for (int w = kNT / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sr[threadIdx.x] += sr[threadIdx.x + w];
+      si[threadIdx.x] += si[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double vr = sr[0] * are - si[0] * aim, vi = sr[0] * aim + si[0] * are;
+    if (CPLX) {
+      d[0] = (R)((acc ? (double)d[0] : 0.0) + vr);
+      d[1] = (R)((acc ? (double)d[1] : 0.0) + vi);
+    } else {
+      d[0] = (R)((acc ? (double)d[0] : 0.0) + vr);
+    }
+  }
+}
+
+// D (+)= (alpha * s) * (conj?) L with the scalar s read from device memory
+// (the rank-0 operand of a contraction, TALSH/tensor_algebra_gpu_nvidia.cu:6755-6795).
+template <typename R, bool CPLX>
+__global__ __launch_bounds__(kNT) void k_axpy_dev(R *__restrict__ d, const R *__restrict__ l,
+                                                   const R *__restrict__ sp, uint64_t n, double are, double aim,
+                                                   bool lconj, bool sconj, bool acc)
+{
+  double fr, fi;
+  if (CPLX) {
+    const double sr = sp[0], si = sconj ? -(double)sp[1] : (double)sp[1];
+    fr = sr * are - si * aim;
+    fi = sr * aim + si * are;
+  } else {
+    // real kinds take the real approximant of the complex factor s*alpha
+    const double pr = (double)sp[0] * are, pi = (double)sp[0] * aim;
+    fr = pr != 0.0 ? sqrt(pr * pr + pi * pi) * (pr > 0 ? 1.0 : -1.0) : pi;
+    fi = 0.0;
+  }
+  const R f0 = (R)fr, f1 = (R)fi;
+  const uint64_t st = (uint64_t)gridDim.x * kNT;
+  for (uint64_t i = (uint64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += st) {
+    if (CPLX) {
+      const R xr = l[2 * i], xi = lconj ? -l[2 * i + 1] : l[2 * i + 1];
+      const R tr = sub(mul(xr, f0), mul(xi, f1)), ti = add(mul(xr, f1), mul(xi, f0));
+      d[2 * i] = acc ? add(d[2 * i], tr) : tr;
+      d[2 * i + 1] = acc ? add(d[2 * i + 1], ti) : ti;
+    } else {
+      const R t = mul(l[i], f0);
+      d[i] = acc ? add(d[i], t) : t;
+    }
+  }
+}
+
+int grid_for(uint64_t n)
+{
+  static int cus = [] {
+    int dev = 0, c = 256;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0)
+      c = p.multiProcessorCount;
+    return c;
+  }();
+  const uint64_t b = (n + kNT - 1) / kNT;
+  const uint64_t cap = (uint64_t)cus * 16;
+  return (int)(b < cap ? (b ? b : 1) : cap);
+}
+
+// Real approximant of a complex factor for real data kinds: modulus with the
+// sign of the real part, or the imaginary part when the real part is zero
+// (the CPU path's rule, TALSH/tensor_algebra_cpu.F90:430-445).
+double real_factor(double re, double im)
+{
+  if (re != 0.0) return std::sqrt(re * re + im * im) * (re > 0 ? 1.0 : -1.0);
+  return im;
+}
+
+}  // namespace
+
+int launch_fill(void *d, int kind, size_t vol, double re, double im, hipStream_t s)
+{
+  if (!vol) return 0;
+  const int g = grid_for(vol);
+  switch (kind) {
+  case R4: hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kNT), 0, s, (float *)d, (uint64_t)vol, (float)real_factor(re, im)); break;
+  case R8: hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kNT), 0, s, (double *)d, (uint64_t)vol, re); break;
+  case C4: hipLaunchKernelGGL(k_fill<float2>, dim3(g), dim3(kNT), 0, s, (float2 *)d, (uint64_t)vol, make_float2((float)re, (float)im)); break;
+  case C8: hipLaunchKernelGGL(k_fill<double2>, dim3(g), dim3(kNT), 0, s, (double2 *)d, (uint64_t)vol, make_double2(re, im)); break;
+  default: return TALSH_INVALID_ARGS;
+  }
+  return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
+}
+
+int launch_scale(void *d, int kind, size_t vol, double re, double im, hipStream_t s)
+{
+  if (!vol) return 0;
+  const int g = grid_for(vol);
+  switch (kind) {
+  case R4: hipLaunchKernelGGL(k_scale_real<float>, dim3(g), dim3(kNT), 0, s, (float *)d, (uint64_t)vol, (float)real_factor(re, im)); break;
+  case R8: hipLaunchKernelGGL(k_scale_real<double>, dim3(g), dim3(kNT), 0, s, (double *)d, (uint64_t)vol, real_factor(re, im)); break;
+  case C4: hipLaunchKernelGGL(k_scale_cplx<float>, dim3(g), dim3(kNT), 0, s, (Cx<float> *)d, (uint64_t)vol, (float)re, (float)im); break;
+  case C8: hipLaunchKernelGGL(k_scale_cplx<double>, dim3(g), dim3(kNT), 0, s, (Cx<double> *)d, (uint64_t)vol, re, im); break;
+  default: return TALSH_INVALID_ARGS;
+  }
+  return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
+}
+
+int launch_axpy(void *d, const void *l, int kind, size_t vol, double re, double im, bool conj, bool acc, hipStream_t s)
+{
+  if (!vol) return 0;
+  const int g = grid_for(vol);
+  const uint64_t n = vol;
+  switch (kind) {
+  case R4: {
+    const float f = (float)real_factor(re, im);
+    if (acc) hipLaunchKernelGGL((k_axpy_real<float, true>), dim3(g), dim3(kNT), 0, s, (float *)d, (const float *)l, n, f);
+    else hipLaunchKernelGGL((k_axpy_real<float, false>), dim3(g), dim3(kNT), 0, s, (float *)d, (const float *)l, n, f);
+    break;
+  }
+  case R8: {
+    const double f = real_factor(re, im);
+    if (acc) hipLaunchKernelGGL((k_axpy_real<double, true>), dim3(g), dim3(kNT), 0, s, (double *)d, (const double *)l, n, f);
+    else hipLaunchKernelGGL((k_axpy_real<double, false>), dim3(g), dim3(kNT), 0, s, (double *)d, (const double *)l, n, f);
+    break;
+  }
+  case C4: {
+    auto *dd = (Cx<float> *)d; auto *ll = (const Cx<float> *)l;
+    const float fr = (float)re, fi = (float)im;
+    if (acc && conj) hipLaunchKernelGGL((k_axpy_cplx<float, true, true>), dim3(g), dim3(kNT), 0, s, dd, ll, n, fr, fi);
+    else if (acc) hipLaunchKernelGGL((k_axpy_cplx<float, true, false>), dim3(g), dim3(kNT), 0, s, dd, ll, n, fr, fi);
+    else if (conj) hipLaunchKernelGGL((k_axpy_cplx<float, false, true>), dim3(g), dim3(kNT), 0, s, dd, ll, n, fr, fi);
+    else hipLaunchKernelGGL((k_axpy_cplx<float, false, false>), dim3(g), dim3(kNT), 0, s, dd, ll, n, fr, fi);
+    break;
+  }
+  case C8: {
+    auto *dd = (Cx<double> *)d; auto *ll = (const Cx<double> *)l;
+    if (acc && conj) hipLaunchKernelGGL((k_axpy_cplx<double, true, true>), dim3(g), dim3(kNT), 0, s, dd, ll, n, re, im);
+    else if (acc) hipLaunchKernelGGL((k_axpy_cplx<double, true, false>), dim3(g), dim3(kNT), 0, s, dd, ll, n, re, im);
+    else if (conj) hipLaunchKernelGGL((k_axpy_cplx<double, false, true>), dim3(g), dim3(kNT), 0, s, dd, ll, n, re, im);
+    else hipLaunchKernelGGL((k_axpy_cplx<double, false, false>), dim3(g), dim3(kNT), 0, s, dd, ll, n, re, im);
+    break;
+  }
+  default: return TALSH_INVALID_ARGS;
+  }
+  return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
+}
+
+int launch_axpy_dev(void *d, const void *l, const void *sp, int kind, size_t vol, double are, double aim, bool lconj,
+                    bool sconj, bool acc, hipStream_t s)
+{
+  if (!vol) return 0;
+  const int g = grid_for(vol);
+  const uint64_t n = vol;
+  switch (kind) {
+  case R4: hipLaunchKernelGGL((k_axpy_dev<float, false>), dim3(g), dim3(kNT), 0, s, (float *)d, (const float *)l, (const float *)sp, n, are, aim, lconj, sconj, acc); break;
+  case R8: hipLaunchKernelGGL((k_axpy_dev<double, false>), dim3(g), dim3(kNT), 0, s, (double *)d, (const double *)l, (const double *)sp, n, are, aim, lconj, sconj, acc); break;
+  case C4: hipLaunchKernelGGL((k_axpy_dev<float, true>), dim3(g), dim3(kNT), 0, s, (float *)d, (const float *)l, (const float *)sp, n, are, aim, lconj, sconj, acc); break;
+  case C8: hipLaunchKernelGGL((k_axpy_dev<double, true>), dim3(g), dim3(kNT), 0, s, (double *)d, (const double *)l, (const double *)sp, n, are, aim, lconj, sconj, acc); break;
+  default: return TALSH_INVALID_ARGS;
+  }
+  return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
+}
+
+int launch_dot(void *d, const void *l, const void *r, int kind, size_t n, double are, double aim, bool lconj,
+               bool rconj, bool acc, void *scratch, size_t scratch_bytes, hipStream_t s)
+{
+  const int np = std::min(grid_for(n), 1024);
+  if (scratch_bytes < (size_t)np * sizeof(double2)) return TALSH_INVALID_ARGS;
+  double2 *part = (double2 *)scratch;
+  const uint64_t nn = n;
+  switch (kind) {
+  case R4:
+    hipLaunchKernelGGL((k_dot_partial<float, false>), dim3(np), dim3(kNT), 0, s, (const float *)l, (const float *)r, nn, false, false, part);
+    hipLaunchKernelGGL((k_dot_final<float, false>), dim3(1), dim3(kNT), 0, s, part, np, (float *)d, are, 0.0, acc);
+    break;
+  case R8:
+    hipLaunchKernelGGL((k_dot_partial<double, false>), dim3(np), dim3(kNT), 0, s, (const double *)l, (const double *)r, nn, false, false, part);
+    hipLaunchKernelGGL((k_dot_final<double, false>), dim3(1), dim3(kNT), 0, s, part, np, (double *)d, are, 0.0, acc);
+    break;
+  case C4:
+    hipLaunchKernelGGL((k_dot_partial<float, true>), dim3(np), dim3(kNT), 0, s, (const float *)l, (const float *)r, nn, lconj, rconj, part);
+    hipLaunchKernelGGL((k_dot_final<float, true>), dim3(1), dim3(kNT), 0, s, part, np, (float *)d, are, aim, acc);
+    break;
+  case C8:
+    hipLaunchKernelGGL((k_dot_partial<double, true>), dim3(np), dim3(kNT), 0, s, (const double *)l, (const double *)r, nn, lconj, rconj, part);
+    hipLaunchKernelGGL((k_dot_final<double, true>), dim3(1), dim3(kNT), 0, s, part, np, (double *)d, are, aim, acc);
+    break;
+  default: return TALSH_INVALID_ARGS;
+  }
+  return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
+}
+
+}  // namespace xt

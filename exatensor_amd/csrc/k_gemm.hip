@@ -1,0 +1,455 @@
+// Matrix-core GEMMs for gfx950: the compute step of a tensor contraction
+// (D(ll,lr) += alpha * L(lc,ll)^T * R(lc,lr) after matricization).
+//
+// Replaces the reference's cuBLAS calls and gpu_matrix_multiply_tn__
+// (TALSH/tensor_algebra_gpu_nvidia.cu:6852-7056, :2263-2312) with in-house
+// kernels:
+//   R8 / C8 : v_mfma_f64_16x16x4_f64, FP64 accumulation;
+//   R4 / C4 : v_mfma_f32_16x16x4_f32, FP32 accumulation ("native FP32");
+//   R4 fast : three-way BF16 split on v_mfma_f32_16x16x32_bf16 (six partial
+//             products per pair, FP32 MFMA accumulation folded into FP64
+//             every few k-steps) -- see k_gemm_bf16x3 below.
+// Operands: A stored k x m with k fastest ("T") or m x k with m fastest ("N");
+// B stored k x n with k fastest ("N") or n x k with n fastest ("T");
+// C is m x n column-major. Complex operands are interleaved (re, im) and are
+// split into real/imaginary LDS planes while staging.
+#include <algorithm>
+
+#include "kernels.hpp"
+
+namespace xt {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Acc;
+template <>
+struct Acc<double> { typedef d4 type; };
+template <>
+struct Acc<float> { typedef f4 type; };
+
+__device__ __forceinline__ d4 mfma(double a, double b, d4 c)
+{
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma(float a, float b, f4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+struct GemmParams {
+  const void *a;
+  const void *b;
+  void *c;
+  int64_t m, n, k;
+  int64_t lda, ldb, ldc;     // leading dims in elements (complex: in complex elements)
+  double alpha_re, alpha_im;
+  int beta_zero;
+  int conj_a, conj_b;
+  int tiles_m, tiles_n;
+  int xcd_remap;             // 1: blockIdx -> tile remap that keeps neighbours on one XCD
+};
+
+// Maps a workgroup to an output tile: first gather the workgroups that the
+// dispatcher places on one XCD (round-robin over 8) into a contiguous range,
+// then walk that range in column groups of 8 M-tiles so neighbouring
+// workgroups share B panels (and A panels within a group) in their XCD's L2.
+__device__ __forceinline__ void tile_of(const GemmParams &p, int &tm, int &tn)
+{
+  const int nb = gridDim.x;
+  int t = blockIdx.x;
+  if (p.xcd_remap && (nb % 8) == 0) t = (t % 8) * (nb / 8) + t / 8;
+  const int G = 8;
+  const int per_group = G * p.tiles_n;
+  const int group = t / per_group;
+  const int first_m = group * G;
+  const int gm = min(p.tiles_m - first_m, G);
+  const int r = t - group * per_group;
+  tm = first_m + r % gm;
+  tn = r / gm;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 / FP32 16x16x4 family. Workgroup: 256 threads = 2x2 waves; each wave
+// owns a (BM/2) x (BN/2) sub-tile of 16x16 MFMA blocks.
+template <typename T, bool CPLX, bool AK, bool BKF, int BM, int BN, int BK>
+__global__ __launch_bounds__(256) void k_gemm_mfma(const GemmParams p)
+{
+  constexpr int NP = CPLX ? 2 : 1;             // LDS planes (re, im)
+  constexpr int PADM = 4, PADN = 4;
+  constexpr int FM = BM / 32, FN = BN / 32;    // MFMA blocks per wave along m / n
+  constexpr int LA = BM * BK / 256;            // A elements staged per thread
+  constexpr int LB = BN * BK / 256;
+  typedef typename Acc<T>::type AccT;
+  __shared__ T As[2][NP][BK][BM + PADM];
+  __shared__ T Bs[2][NP][BK][BN + PADN];
+
+  int tm, tn;
+  tile_of(p, tm, tn);
+  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * (BN / 2);
+  const T *A = static_cast<const T *>(p.a);
+  const T *B = static_cast<const T *>(p.b);
+
+  // staging coordinates: for k-fast operands a thread covers LA consecutive k
+  // of one row; for row-fast operands LA consecutive rows of one k.
+  int a_r, a_k, b_r, b_k;
+  if (AK) { a_r = tid / (BK / LA); a_k = (tid % (BK / LA)) * LA; }
+  else { a_k = tid / (BM / LA); a_r = (tid % (BM / LA)) * LA; }
+  if (BKF) { b_r = tid / (BK / LB); b_k = (tid % (BK / LB)) * LB; }
+  else { b_k = tid / (BN / LB); b_r = (tid % (BN / LB)) * LB; }
+
+  T ra[NP][LA], rb[NP][LB];
+  auto load_a = [&](int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < LA; ++i) {
+      const int64_t r = m0 + a_r + (AK ? 0 : i), kk = k0 + a_k + (AK ? i : 0);
+      const bool ok = r < p.m && kk < p.k;
+      const int64_t off = AK ? (r * p.lda + kk) : (kk * p.lda + r);
+      if (CPLX) {
+        ra[0][i] = ok ? A[2 * off] : T(0);
+        const T im = ok ? A[2 * off + 1] : T(0);
+        ra[1 % NP][i] = p.conj_a ? -im : im;
+      } else {
+        ra[0][i] = ok ? A[off] : T(0);
+      }
+    }
+  };
+  auto load_b = [&](int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+      const int64_t r = n0 + b_r + (BKF ? 0 : i), kk = k0 + b_k + (BKF ? i : 0);
+      const bool ok = r < p.n && kk < p.k;
+      const int64_t off = BKF ? (r * p.ldb + kk) : (kk * p.ldb + r);
+      if (CPLX) {
+        rb[0][i] = ok ? B[2 * off] : T(0);
+        const T im = ok ? B[2 * off + 1] : T(0);
+        rb[1 % NP][i] = p.conj_b ? -im : im;
+      } else {
+        rb[0][i] = ok ? B[off] : T(0);
+      }
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+#pragma unroll
+      for (int i = 0; i < LA; ++i) {
+        if (AK) As[buf][q][a_k + i][a_r] = ra[q][i];
+        else As[buf][q][a_k][a_r + i] = ra[q][i];
+      }
+#pragma unroll
+      for (int i = 0; i < LB; ++i) {
+        if (BKF) Bs[buf][q][b_k + i][b_r] = rb[q][i];
+        else Bs[buf][q][b_k][b_r + i] = rb[q][i];
+      }
+    }
+  };
+
+  AccT acc[NP][FM][FN];
+#pragma unroll
+  for (int q = 0; q < NP; ++q)
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[q][i][j] = AccT{0, 0, 0, 0};
+
+  const int64_t nk = (p.k + BK - 1) / BK;
+  load_a(0);
+  load_b(0);
+  store(0);
+  __syncthreads();
+  const int fr = lane & 15, fk = lane >> 4;
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) {
+      load_a((kt + 1) * BK);
+      load_b((kt + 1) * BK);
+    }
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      T fa[NP][FM], fb[NP][FN];
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i) fa[q][i] = As[cur][q][ks + fk][wm + i * 16 + fr];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) fb[q][j] = Bs[cur][q][ks + fk][wn + j * 16 + fr];
+      }
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          if (CPLX) {
+            // (ar + i ai)(br + i bi) = (ar br - ai bi) + i (ar bi + ai br)
+            acc[0][i][j] = mfma(fa[0][i], fb[0][j], acc[0][i][j]);
+            acc[0][i][j] = mfma(-fa[1 % NP][i], fb[1 % NP][j], acc[0][i][j]);
+            acc[1 % NP][i][j] = mfma(fa[0][i], fb[1 % NP][j], acc[1 % NP][i][j]);
+            acc[1 % NP][i][j] = mfma(fa[1 % NP][i], fb[0][j], acc[1 % NP][i][j]);
+          } else {
+            acc[0][i][j] = mfma(fa[0][i], fb[0][j], acc[0][i][j]);
+          }
+        }
+      }
+    }
+    if (kt + 1 < nk) store(cur ^ 1);
+    __syncthreads();
+  }
+  // epilogue: C = alpha * acc + beta * C (beta 0 or 1)
+  T *C = static_cast<T *>(p.c);
+  const T ar = (T)p.alpha_re, ai = (T)p.alpha_im;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int64_t col = n0 + wn + j * 16 + fr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        // accumulator layout: FP32 16x16x4 keeps rows 4*(lane/16)+r in a
+        // lane, FP64 16x16x4 keeps rows lane/16 + 4*r
+        const int64_t row = m0 + wm + i * 16 + (sizeof(T) == 8 ? fk + 4 * r : fk * 4 + r);
+        if (row < p.m && col < p.n) {
+          const int64_t off = row + col * p.ldc;
+          if (CPLX) {
+            const T xr = acc[0][i][j][r], xi = acc[1 % NP][i][j][r];
+            T vr = ar * xr - ai * xi, vi = ar * xi + ai * xr;
+            if (!p.beta_zero) { vr += C[2 * off]; vi += C[2 * off + 1]; }
+            C[2 * off] = vr;
+            C[2 * off + 1] = vi;
+          } else {
+            T v = ar * acc[0][i][j][r];
+            if (!p.beta_zero) v += C[off];
+            C[off] = v;
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Three-way BF16 split for FP32 operands (Henry, Tang & Heinecke, "Leveraging
+// the bfloat16 artificial intelligence datatype for higher-precision
+// computations", ARITH 2019): x = b0 + b1 + b2 exactly (bf16 each, RNE);
+// x*y ~= b0c0 + (b0c1 + b1c0) + (b0c2 + b1c1 + b2c0), dropping terms below
+// 2^-24 relative. Each product runs on v_mfma_f32_16x16x32_bf16; the FP32
+// accumulator is folded into an FP64 sum every kFold k-tiles so the
+// accumulation error stays at the level of a short FP32 sum.
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ unsigned short bf16_rne(float x)
+{
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ void split3(float x, unsigned short &h0, unsigned short &h1, unsigned short &h2)
+{
+  h0 = bf16_rne(x);
+  const float r1 = x - bf16_to_f(h0);
+  h1 = bf16_rne(r1);
+  const float r2 = r1 - bf16_to_f(h1);
+  h2 = bf16_rne(r2);
+}
+
+__device__ __forceinline__ f4 mfma_bf(const us8 &a, const us8 &b, f4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+}
+
+template <bool AK, bool BKF, int BM, int BN>
+__global__ __launch_bounds__(256) void k_gemm_bf16x3(const GemmParams p)
+{
+  constexpr int BK = 32;                       // one MFMA k-step per LDS tile
+  constexpr int KP = BK + 8;                   // padded k extent of an LDS row (bf16)
+  constexpr int FM = BM / 32, FN = BN / 32;
+  constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
+  constexpr int kFold = 8;                     // k-tiles between FP64 folds
+  __shared__ __align__(16) unsigned short As[2][3][BM][KP];
+  __shared__ __align__(16) unsigned short Bs[2][3][BN][KP];
+
+  int tm, tn;
+  tile_of(p, tm, tn);
+  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * (BN / 2);
+  const float *A = static_cast<const float *>(p.a);
+  const float *B = static_cast<const float *>(p.b);
+  int a_r, a_k, b_r, b_k;
+  if (AK) { a_r = tid / (BK / LA); a_k = (tid % (BK / LA)) * LA; }
+  else { a_k = tid / (BM / LA); a_r = (tid % (BM / LA)) * LA; }
+  if (BKF) { b_r = tid / (BK / LB); b_k = (tid % (BK / LB)) * LB; }
+  else { b_k = tid / (BN / LB); b_r = (tid % (BN / LB)) * LB; }
+  float ra[LA], rb[LB];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < LA; ++i) {
+      const int64_t r = m0 + a_r + (AK ? 0 : i), kk = k0 + a_k + (AK ? i : 0);
+      ra[i] = (r < p.m && kk < p.k) ? A[AK ? (r * p.lda + kk) : (kk * p.lda + r)] : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+      const int64_t r = n0 + b_r + (BKF ? 0 : i), kk = k0 + b_k + (BKF ? i : 0);
+      rb[i] = (r < p.n && kk < p.k) ? B[BKF ? (r * p.ldb + kk) : (kk * p.ldb + r)] : 0.f;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < LA; ++i) {
+      unsigned short h0, h1, h2;
+      split3(ra[i], h0, h1, h2);
+      const int r = a_r + (AK ? 0 : i), kk = a_k + (AK ? i : 0);
+      As[buf][0][r][kk] = h0; As[buf][1][r][kk] = h1; As[buf][2][r][kk] = h2;
+    }
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+      unsigned short h0, h1, h2;
+      split3(rb[i], h0, h1, h2);
+      const int r = b_r + (BKF ? 0 : i), kk = b_k + (BKF ? i : 0);
+      Bs[buf][0][r][kk] = h0; Bs[buf][1][r][kk] = h1; Bs[buf][2][r][kk] = h2;
+    }
+  };
+  f4 acc[FM][FN];
+  double sum[FM][FN][4];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      acc[i][j] = f4{0, 0, 0, 0};
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sum[i][j][r] = 0.0;
+    }
+  const int64_t nk = (p.k + BK - 1) / BK;
+  load(0);
+  store(0);
+  __syncthreads();
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * BK);
+    us8 fa[3][FM], fb[3][FN];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i) fa[q][i] = *reinterpret_cast<const us8 *>(&As[cur][q][wm + i * 16 + fr][fk]);
+#pragma unroll
+      for (int j = 0; j < FN; ++j) fb[q][j] = *reinterpret_cast<const us8 *>(&Bs[cur][q][wn + j * 16 + fr][fk]);
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        // smallest terms first
+        acc[i][j] = mfma_bf(fa[0][i], fb[2][j], acc[i][j]);
+        acc[i][j] = mfma_bf(fa[1][i], fb[1][j], acc[i][j]);
+        acc[i][j] = mfma_bf(fa[2][i], fb[0][j], acc[i][j]);
+        acc[i][j] = mfma_bf(fa[0][i], fb[1][j], acc[i][j]);
+        acc[i][j] = mfma_bf(fa[1][i], fb[0][j], acc[i][j]);
+        acc[i][j] = mfma_bf(fa[0][i], fb[0][j], acc[i][j]);
+      }
+    }
+    if ((kt + 1) % kFold == 0 || kt + 1 == nk) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) sum[i][j][r] += (double)acc[i][j][r];
+          acc[i][j] = f4{0, 0, 0, 0};
+        }
+    }
+    if (kt + 1 < nk) store(cur ^ 1);
+    __syncthreads();
+  }
+  float *C = static_cast<float *>(p.c);
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int64_t col = n0 + wn + j * 16 + fr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+        if (row < p.m && col < p.n) {
+          const int64_t off = row + col * p.ldc;
+          double v = p.alpha_re * sum[i][j][r];
+          if (!p.beta_zero) v += (double)C[off];
+          C[off] = (float)v;
+        }
+      }
+    }
+}
+
+template <typename K>
+int launch_grid(K kern, const GemmParams &p, hipStream_t s)
+{
+  const int nb = p.tiles_m * p.tiles_n;
+  hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s, p);
+  return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
+}
+
+template <typename T, bool CPLX, int BM, int BN, int BK>
+int launch_family(GemmParams p, bool ak, bool bk, hipStream_t s)
+{
+  p.tiles_m = (int)((p.m + BM - 1) / BM);
+  p.tiles_n = (int)((p.n + BN - 1) / BN);
+  if (ak && bk) return launch_grid(k_gemm_mfma<T, CPLX, true, true, BM, BN, BK>, p, s);
+  if (ak) return launch_grid(k_gemm_mfma<T, CPLX, true, false, BM, BN, BK>, p, s);
+  if (bk) return launch_grid(k_gemm_mfma<T, CPLX, false, true, BM, BN, BK>, p, s);
+  return launch_grid(k_gemm_mfma<T, CPLX, false, false, BM, BN, BK>, p, s);
+}
+
+template <int BM, int BN>
+int launch_bf16x3(GemmParams p, bool ak, bool bk, hipStream_t s)
+{
+  p.tiles_m = (int)((p.m + BM - 1) / BM);
+  p.tiles_n = (int)((p.n + BN - 1) / BN);
+  if (ak && bk) return launch_grid(k_gemm_bf16x3<true, true, BM, BN>, p, s);
+  if (ak) return launch_grid(k_gemm_bf16x3<true, false, BM, BN>, p, s);
+  if (bk) return launch_grid(k_gemm_bf16x3<false, true, BM, BN>, p, s);
+  return launch_grid(k_gemm_bf16x3<false, false, BM, BN>, p, s);
+}
+
+}  // namespace
+
+size_t gemm_workspace_bytes(const GemmArgs &) { return 0; }
+
+int launch_gemm_ws(const GemmArgs &g, void *, hipStream_t s) { return launch_gemm(g, s); }
+
+int launch_gemm(const GemmArgs &g, hipStream_t s)
+{
+  if (g.m == 0 || g.n == 0) return 0;
+  GemmParams p{};
+  p.a = g.a; p.b = g.b; p.c = g.c;
+  p.m = (int64_t)g.m; p.n = (int64_t)g.n; p.k = (int64_t)g.k;
+  p.lda = g.a_kfast ? p.k : p.m;
+  p.ldb = g.b_kfast ? p.k : p.n;
+  p.ldc = p.m;
+  p.alpha_re = g.alpha_re; p.alpha_im = g.alpha_im;
+  p.beta_zero = g.beta_zero ? 1 : 0;
+  p.conj_a = g.conj_a; p.conj_b = g.conj_b;
+  p.xcd_remap = 1;
+  // tile counts must fit the 1-D grid
+  if ((p.m + 63) / 64 * ((p.n + 63) / 64) >= (1ll << 31)) return TALSH_LIMIT_EXCEEDED;
+  if (g.k == 0) {
+    // empty contraction: C = beta * C
+    if (!g.beta_zero) return 0;
+  }
+  switch (g.kind) {
+  case R8: return launch_family<double, false, 128, 128, 16>(p, g.a_kfast, g.b_kfast, s);
+  case C8: return launch_family<double, true, 64, 64, 16>(p, g.a_kfast, g.b_kfast, s);
+  case R4:
+    if (g.r4mode == R4Mode::kBF16x3) return launch_bf16x3<128, 128>(p, g.a_kfast, g.b_kfast, s);
+    return launch_family<float, false, 128, 128, 16>(p, g.a_kfast, g.b_kfast, s);
+  case C4: return launch_family<float, true, 64, 64, 16>(p, g.a_kfast, g.b_kfast, s);
+  default: return TALSH_INVALID_ARGS;
+  }
+}
+
+}  // namespace xt

@@ -1,0 +1,595 @@
+// DLF tensor permutation for gfx950 (the engine's transpose/copy kernel).
+//
+// Replaces the reference's gpu_tensor_block_copy_dlf__ /
+// gpu_tensor_block_copy_scatter_dlf__ (TALSH/tensor_algebra_gpu_nvidia.cu:1125-1457,
+// :2172-2260). Semantics: out[idx'] = in[idx], input dimension i becoming output
+// dimension o2n[i]; pure data movement, bit-exact.
+//
+// Design:
+//  * host planner: drop unit extents, fuse input dimensions that stay adjacent
+//    and in order in the output, then pick an LDS tile = the union of an
+//    input-minor dimension group and an output-minor dimension group, grown
+//    until both contiguous runs reach >= 256 bytes (the last dimension of each
+//    group may be split; ragged splits are masked);
+//  * LDS padding is chosen on the host by simulating the bank conflicts of the
+//    store phase for each candidate padding period;
+//  * persistent kernel: each thread decodes the intra-tile offsets of its
+//    elements once (input offset, output offset, LDS slots) and keeps them in
+//    registers, so the per-tile work is one base-offset decode per workgroup
+//    plus one coalesced load and one coalesced store per element.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include "kernels.hpp"
+
+namespace xt {
+namespace {
+
+constexpr int kNT = 256;           // threads per workgroup
+constexpr int kMaxTileDims = 8;    // dimensions inside one tile
+constexpr int kMaxGridDims = 16;   // fused rank handled by the tiled path
+constexpr int kMaxEPT = 16;        // tile elements per thread
+
+struct DevDiv { uint32_t d, m, s; };
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const DevDiv &f)
+{
+  // d == 0 encodes "no division" (result 0); d == 1 is the identity
+  if (f.d == 0) return 0;
+  return (f.d == 1) ? n : ((__umulhi(n, f.m) + n) >> f.s);
+}
+
+struct TileParams {
+  // tile dimensions in input (load) order
+  int nt;
+  DevDiv t_in[kMaxTileDims];       // tile extents, load order
+  uint64_t is_in[kMaxTileDims];    // input strides, load order
+  // tile dimensions in output (store) order
+  DevDiv t_out[kMaxTileDims];      // tile extents, store order
+  uint64_t os_out[kMaxTileDims];   // output strides, store order
+  uint32_t ls_out[kMaxTileDims];   // LDS (load-order linear) strides, store order
+  // ragged (partially tiled) dimensions: position in load / store order
+  int nrag;
+  int rag_in[2], rag_out[2];
+  uint32_t rag_t[2];
+  // grid over all fused dims
+  int ng;
+  DevDiv gdiv[kMaxGridDims];       // number of tiles along each dim
+  uint64_t g_is[kMaxGridDims];     // input stride * tile extent
+  uint64_t g_os[kMaxGridDims];     // output stride * tile extent
+  uint32_t g_t[kMaxGridDims];      // tile extent along the dim
+  uint32_t g_ext[kMaxGridDims];    // full extent
+  int g_rag[kMaxGridDims];         // ragged slot (0/1) or -1
+  uint32_t vol_tile;               // elements per tile
+  uint32_t ntiles;
+  DevDiv pad;                      // LDS slot = e + e / pad.d (pad.d = 0: no padding)
+};
+
+template <typename T, bool CONJ>
+__device__ __forceinline__ T conj_if(T v) { return v; }
+template <>
+__device__ __forceinline__ float2 conj_if<float2, true>(float2 v) { v.y = -v.y; return v; }
+template <>
+__device__ __forceinline__ double2 conj_if<double2, true>(double2 v) { v.y = -v.y; return v; }
+
+template <typename T, int EPT, bool CONJ, typename OffT, bool RAG>
+__global__ __launch_bounds__(kNT) void k_permute_tiled(const T *__restrict__ in, T *__restrict__ out,
+                                                        const TileParams p)
+{
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *lds = reinterpret_cast<T *>(lds_raw);
+  const uint32_t tid = threadIdx.x;
+  OffT in_off[EPT], out_off[EPT];
+  uint32_t st_slot[EPT];
+  uint32_t rin[RAG ? EPT : 1], rout[RAG ? EPT : 1];
+  // one-time decode of this thread's tile elements (kept in registers)
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const uint32_t e = tid + (uint32_t)k * kNT;
+    uint32_t r = e, cin[2] = {0, 0}, cout[2] = {0, 0};
+    OffT io = 0, oo = 0;
+    uint32_t ls = 0;
+    for (int j = 0; j < p.nt; ++j) {
+      const uint32_t q = fdiv(r, p.t_in[j]);
+      const uint32_t c = r - q * p.t_in[j].d;
+      r = q;
+      io += (OffT)c * (OffT)p.is_in[j];
+      if (RAG) {
+        if (p.rag_in[0] == j) cin[0] = c;
+        if (p.nrag > 1 && p.rag_in[1] == j) cin[1] = c;
+      }
+    }
+    r = e;
+    for (int j = 0; j < p.nt; ++j) {
+      const uint32_t q = fdiv(r, p.t_out[j]);
+      const uint32_t c = r - q * p.t_out[j].d;
+      r = q;
+      oo += (OffT)c * (OffT)p.os_out[j];
+      ls += c * p.ls_out[j];
+      if (RAG) {
+        if (p.rag_out[0] == j) cout[0] = c;
+        if (p.nrag > 1 && p.rag_out[1] == j) cout[1] = c;
+      }
+    }
+    in_off[k] = io;
+    out_off[k] = oo;
+    st_slot[k] = ls + fdiv(ls, p.pad);
+    if (RAG) {
+      const bool valid = e < p.vol_tile;
+      rin[k] = valid ? (cin[0] | (cin[1] << 16)) : 0xFFFFFFFFu;
+      rout[k] = valid ? (cout[0] | (cout[1] << 16)) : 0xFFFFFFFFu;
+    }
+  }
+  for (uint32_t tile = blockIdx.x; tile < p.ntiles; tile += gridDim.x) {
+    // decode the tile's base offsets (uniform across the workgroup)
+    OffT bin = 0, bout = 0;
+    uint32_t rem[2] = {0xFFFFu, 0xFFFFu};
+    uint32_t r = tile;
+    for (int g = 0; g < p.ng; ++g) {
+      const uint32_t q = fdiv(r, p.gdiv[g]);
+      const uint32_t c = r - q * p.gdiv[g].d;
+      r = q;
+      bin += (OffT)c * (OffT)p.g_is[g];
+      bout += (OffT)c * (OffT)p.g_os[g];
+      if (RAG && p.g_rag[g] >= 0) {
+        const uint32_t left = p.g_ext[g] - c * p.g_t[g];
+        rem[p.g_rag[g]] = left < p.g_t[g] ? left : p.g_t[g];
+      }
+    }
+    const bool full = !RAG || ((rem[0] == p.rag_t[0]) && (p.nrag < 2 || rem[1] == p.rag_t[1]));
+    T v[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const uint32_t e = tid + (uint32_t)k * kNT;
+      bool ok = e < p.vol_tile;
+      if (RAG && !full) ok = ok && (rin[k] & 0xFFFFu) < rem[0] && (rin[k] >> 16) < rem[1];
+      if (ok) v[k] = in[bin + in_off[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const uint32_t e = tid + (uint32_t)k * kNT;
+      bool ok = e < p.vol_tile;
+      if (RAG && !full) ok = ok && (rin[k] & 0xFFFFu) < rem[0] && (rin[k] >> 16) < rem[1];
+      if (ok) lds[e + fdiv(e, p.pad)] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const uint32_t f = tid + (uint32_t)k * kNT;
+      bool ok = f < p.vol_tile;
+      if (RAG && !full) ok = ok && (rout[k] & 0xFFFFu) < rem[0] && (rout[k] >> 16) < rem[1];
+      if (ok) out[bout + out_off[k]] = conj_if<T, CONJ>(lds[st_slot[k]]);
+    }
+    __syncthreads();
+  }
+}
+
+// Generic per-element fallback (any rank): decode the input index, scatter.
+struct ScatterParams {
+  int rank;
+  uint64_t vol;
+  uint64_t ext[kMaxRank];
+  uint64_t ostr[kMaxRank];
+};
+
+template <typename T, bool CONJ>
+__global__ __launch_bounds__(kNT) void k_permute_scatter(const T *__restrict__ in, T *__restrict__ out,
+                                                          const ScatterParams p)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * kNT;
+  for (uint64_t i = (uint64_t)blockIdx.x * kNT + threadIdx.x; i < p.vol; i += stride) {
+    uint64_t r = i, o = 0;
+    for (int d = 0; d < p.rank; ++d) {
+      const uint64_t q = r / p.ext[d];
+      o += (r - q * p.ext[d]) * p.ostr[d];
+      r = q;
+    }
+    out[o] = conj_if<T, CONJ>(in[i]);
+  }
+}
+
+template <typename T, bool CONJ>
+__global__ __launch_bounds__(kNT) void k_copy(const T *__restrict__ in, T *__restrict__ out, uint64_t n)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * kNT;
+  for (uint64_t i = (uint64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += stride) out[i] = conj_if<T, CONJ>(in[i]);
+}
+
+// ---------------------------------------------------------------------------
+// host planning
+
+struct Fused {
+  int rank = 0;
+  std::vector<int> ext;   // fused extents, input order
+  std::vector<int> o2n;   // output position of each fused dim
+};
+
+Fused fuse_dims(int rank, const int *dims, const int *o2n)
+{
+  // keep non-unit dims, re-rank their output positions
+  std::vector<int> keep;
+  for (int i = 0; i < rank; ++i)
+    if (dims[i] > 1) keep.push_back(i);
+  std::vector<int> order(keep.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return o2n[keep[a]] < o2n[keep[b]]; });
+  std::vector<int> pos(keep.size());
+  for (size_t q = 0; q < order.size(); ++q) pos[order[q]] = (int)q;
+  // fuse runs i, i+1 with pos[i+1] == pos[i] + 1
+  Fused f;
+  std::vector<int> grp_pos;
+  for (size_t i = 0; i < keep.size(); ++i) {
+    if (i > 0 && pos[i] == pos[i - 1] + 1) {
+      f.ext.back() *= dims[keep[i]];
+    } else {
+      f.ext.push_back(dims[keep[i]]);
+      grp_pos.push_back(pos[i]);
+    }
+  }
+  f.rank = (int)f.ext.size();
+  std::vector<int> ord(f.rank);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) { return grp_pos[a] < grp_pos[b]; });
+  f.o2n.assign(f.rank, 0);
+  for (int q = 0; q < f.rank; ++q) f.o2n[ord[q]] = q;
+  return f;
+}
+
+// Smallest divisor of ext in [need, 2*need], else need itself.
+int pick_split(int ext, int need)
+{
+  for (int t = need; t <= std::min(ext, 2 * need); ++t)
+    if (ext % t == 0) return t;
+  return std::min(ext, need);
+}
+
+struct TilePlan {
+  std::vector<int> t;   // tile extent per fused dim (0: not a tile dim)
+  long vol = 0;
+  int in_run = 0, out_run = 0;
+};
+
+TilePlan choose_tile(const Fused &f, int target_in, int target_out)
+{
+  const int r = f.rank;
+  std::vector<int> n2o(r);
+  for (int i = 0; i < r; ++i) n2o[f.o2n[i]] = i;
+  TilePlan tp;
+  tp.t.assign(r, 0);
+  long run = 1;
+  for (int i = 0; i < r; ++i) {
+    const long need = (target_in + run - 1) / run;
+    if (need >= f.ext[i]) {
+      tp.t[i] = f.ext[i];
+      run *= f.ext[i];
+      if (run >= target_in) break;
+    } else {
+      tp.t[i] = pick_split(f.ext[i], (int)need);
+      run *= tp.t[i];
+      break;
+    }
+  }
+  tp.in_run = (int)std::min<long>(run, 1 << 30);
+  run = 1;
+  for (int q = 0; q < r; ++q) {
+    const int i = n2o[q];
+    const long need = (target_out + run - 1) / run;
+    if (tp.t[i] > 0 && tp.t[i] < f.ext[i] && tp.t[i] < need) tp.t[i] = pick_split(f.ext[i], (int)std::min<long>(need, f.ext[i]));
+    if (tp.t[i] == 0) tp.t[i] = need >= f.ext[i] ? f.ext[i] : pick_split(f.ext[i], (int)need);
+    run *= tp.t[i];
+    if (tp.t[i] < f.ext[i] || run >= target_out) break;
+  }
+  tp.out_run = (int)std::min<long>(run, 1 << 30);
+  // the input run may have grown through the output walk
+  run = 1;
+  for (int i = 0; i < r; ++i) {
+    if (tp.t[i] == 0) break;
+    run *= tp.t[i];
+    if (tp.t[i] < f.ext[i]) break;
+  }
+  tp.in_run = (int)std::min<long>(run, 1 << 30);
+  tp.vol = 1;
+  for (int i = 0; i < r; ++i)
+    if (tp.t[i]) tp.vol *= tp.t[i];
+  return tp;
+}
+
+// LDS bank-conflict degree of the store phase for a padding period.
+int conflict_cost(const std::vector<uint32_t> &slots, int words_per_elem, uint32_t period)
+{
+  int cost = 0;
+  for (int banks : {32, 64}) {
+    for (size_t base = 0; base + 1 <= slots.size(); base += 64) {
+      std::vector<std::vector<uint32_t>> hit(banks);
+      for (size_t l = base; l < std::min(slots.size(), base + 64); ++l) {
+        const uint32_t s = slots[l];
+        const uint32_t slot = period ? s + s / period : s;
+        for (int w = 0; w < words_per_elem; ++w) {
+          const uint32_t word = slot * words_per_elem + w;
+          auto &v = hit[word % banks];
+          if (std::find(v.begin(), v.end(), word) == v.end()) v.push_back(word);
+        }
+      }
+      int mx = 0;
+      for (auto &v : hit) mx = std::max<int>(mx, (int)v.size());
+      cost += mx;
+    }
+  }
+  return cost;
+}
+
+DevDiv to_dev(uint32_t d)
+{
+  if (d == 0) return DevDiv{0, 0, 0};  // "no division" marker
+  FastDiv f(d);
+  return DevDiv{f.d, f.m, f.s};
+}
+
+struct Planned {
+  int path = 0;  // 0 copy, 1 tiled, 2 scatter
+  Fused f;
+  TileParams tp{};
+  int ept = 0;
+  size_t lds_bytes = 0;
+  int grid = 0;
+  bool wide = false;  // 64-bit element offsets
+  bool rag = false;   // masked edge tiles
+  TilePlan tile;
+};
+
+int num_cus_current()
+{
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;
+  }
+  return cus;
+}
+
+int plan(int es, int rank, const int *dims, const int *o2n, Planned &pl, int cus)
+{
+  pl.f = fuse_dims(rank, dims, o2n);
+  const Fused &f = pl.f;
+  bool identity = true;
+  for (int i = 0; i < f.rank; ++i) identity = identity && f.o2n[i] == i;
+  if (f.rank <= 1 || identity) { pl.path = 0; return 0; }
+  if (f.rank > kMaxGridDims) { pl.path = 2; return 0; }
+  uint64_t vol = 1;
+  for (int e : f.ext) vol *= (uint64_t)e;
+  const int max_tile_bytes = 32768;
+  int target = std::max(8, 256 / es);
+  TilePlan tp;
+  while (true) {
+    tp = choose_tile(f, target, target);
+    int ntd = 0;
+    for (int x : tp.t) ntd += x > 0;
+    if ((tp.vol * es <= max_tile_bytes && tp.vol <= (long)kNT * kMaxEPT && ntd <= kMaxTileDims) || target <= 1) break;
+    target = std::max(1, target / 2);
+  }
+  int ntd = 0, nrag = 0;
+  for (int i = 0; i < f.rank; ++i) {
+    if (tp.t[i] > 0) ++ntd;
+    if (tp.t[i] > 0 && f.ext[i] % tp.t[i] != 0) ++nrag;
+  }
+  uint64_t ntiles = 1;
+  for (int i = 0; i < f.rank; ++i) ntiles *= (uint64_t)((f.ext[i] + std::max(tp.t[i], 1) - 1) / std::max(tp.t[i], 1));
+  if (tp.vol * es > max_tile_bytes || ntd > kMaxTileDims || nrag > 2 || ntiles >= (1ull << 31) ||
+      tp.vol > (long)kNT * kMaxEPT) {
+    pl.path = 2;
+    return 0;
+  }
+  pl.path = 1;
+  pl.tile = tp;
+  TileParams &P = pl.tp;
+  std::memset(&P, 0, sizeof(P));
+  // strides
+  std::vector<uint64_t> istr(f.rank), ostr_out(f.rank), ostr(f.rank);
+  uint64_t s = 1;
+  for (int i = 0; i < f.rank; ++i) { istr[i] = s; s *= (uint64_t)f.ext[i]; }
+  std::vector<int> n2o(f.rank);
+  for (int i = 0; i < f.rank; ++i) n2o[f.o2n[i]] = i;
+  s = 1;
+  for (int q = 0; q < f.rank; ++q) { ostr_out[q] = s; s *= (uint64_t)f.ext[n2o[q]]; }
+  for (int i = 0; i < f.rank; ++i) ostr[i] = ostr_out[f.o2n[i]];
+  // tile dims in load order
+  std::vector<int> tin, tout;
+  for (int i = 0; i < f.rank; ++i)
+    if (tp.t[i] > 0) tin.push_back(i);
+  for (int q = 0; q < f.rank; ++q)
+    if (tp.t[n2o[q]] > 0) tout.push_back(n2o[q]);
+  P.nt = (int)tin.size();
+  std::vector<uint32_t> lstride(f.rank, 0);
+  uint32_t ls = 1;
+  for (int j = 0; j < P.nt; ++j) {
+    P.t_in[j] = to_dev((uint32_t)tp.t[tin[j]]);
+    P.is_in[j] = istr[tin[j]];
+    lstride[tin[j]] = ls;
+    ls *= (uint32_t)tp.t[tin[j]];
+  }
+  for (int j = 0; j < P.nt; ++j) {
+    P.t_out[j] = to_dev((uint32_t)tp.t[tout[j]]);
+    P.os_out[j] = ostr[tout[j]];
+    P.ls_out[j] = lstride[tout[j]];
+  }
+  P.nrag = 0;
+  for (int j = 0; j < P.nt; ++j) {
+    const int i = tin[j];
+    if (f.ext[i] % tp.t[i] != 0) {
+      const int slot = P.nrag++;
+      P.rag_in[slot] = j;
+      P.rag_t[slot] = (uint32_t)tp.t[i];
+      for (int jj = 0; jj < P.nt; ++jj)
+        if (tout[jj] == i) P.rag_out[slot] = jj;
+    }
+  }
+  // grid: walk input dims in order (tile index decoded fastest along dim 0)
+  P.ng = f.rank;
+  for (int i = 0; i < f.rank; ++i) {
+    const uint32_t t = (uint32_t)std::max(tp.t[i], 1);
+    const uint32_t n = (uint32_t)((f.ext[i] + t - 1) / t);
+    P.gdiv[i] = to_dev(n);
+    P.g_is[i] = istr[i] * t;
+    P.g_os[i] = ostr[i] * t;
+    P.g_t[i] = t;
+    P.g_ext[i] = (uint32_t)f.ext[i];
+    P.g_rag[i] = -1;
+    for (int k = 0; k < P.nrag; ++k)
+      if (tin[P.rag_in[k]] == i) P.g_rag[i] = k;
+  }
+  P.vol_tile = (uint32_t)tp.vol;
+  P.ntiles = (uint32_t)ntiles;
+  // LDS padding by simulated store-phase conflicts (first 256 store slots)
+  std::vector<uint32_t> st;
+  for (uint32_t e = 0; e < std::min<uint32_t>(P.vol_tile, kNT); ++e) {
+    uint32_t r = e, l = 0;
+    for (int j = 0; j < P.nt; ++j) {
+      const uint32_t d = P.t_out[j].d;
+      l += (r % d) * P.ls_out[j];
+      r /= d;
+    }
+    st.push_back(l);
+  }
+  const int words = std::max(1, es / 4);
+  std::vector<uint32_t> cands = {0, 4, 8, 16, 32, 64, 128, 256};
+  uint32_t prod = 1;
+  for (int j = 0; j < P.nt && j < 3; ++j) { prod *= P.t_in[j].d; cands.push_back(prod); }
+  uint32_t best = 0;
+  int bestc = conflict_cost(st, words, 0);
+  for (uint32_t c : cands) {
+    if (c == 0 || c >= P.vol_tile) continue;
+    const int cc = conflict_cost(st, words, c);
+    if (cc < bestc) { bestc = cc; best = c; }
+  }
+  P.pad = to_dev(best);
+  int ept = 1;
+  while ((long)ept * kNT < tp.vol) ept *= 2;
+  pl.ept = ept;
+  pl.wide = vol >= (1ull << 32);
+  pl.rag = P.nrag > 0;
+  const size_t slots = P.vol_tile + (best ? P.vol_tile / best + 1 : 0);
+  pl.lds_bytes = slots * (size_t)es;
+  const int per_cu = std::max(1, std::min(8, (int)(160 * 1024 / std::max<size_t>(pl.lds_bytes, 1))));
+  pl.grid = (int)std::min<uint64_t>(ntiles, (uint64_t)cus * per_cu);
+  return 0;
+}
+
+template <typename T, bool CONJ, typename OffT, bool RAG>
+int launch_tiled_v(const Planned &pl, const void *in, void *out, hipStream_t s)
+{
+  const T *a = static_cast<const T *>(in);
+  T *b = static_cast<T *>(out);
+  dim3 g(pl.grid), blk(kNT);
+  switch (pl.ept) {
+#define XT_CASE(E)                                                                                       \
+  case E:                                                                                                \
+    hipLaunchKernelGGL((k_permute_tiled<T, E, CONJ, OffT, RAG>), g, blk, pl.lds_bytes, s, a, b, pl.tp);  \
+    break;
+    XT_CASE(1) XT_CASE(2) XT_CASE(4) XT_CASE(8) XT_CASE(16)
+#undef XT_CASE
+  default: return TALSH_FAILURE;
+  }
+  return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
+}
+
+template <typename T, bool CONJ>
+int launch_tiled_t(const Planned &pl, const void *in, void *out, hipStream_t s)
+{
+  if (pl.wide) return pl.rag ? launch_tiled_v<T, CONJ, uint64_t, true>(pl, in, out, s)
+                             : launch_tiled_v<T, CONJ, uint64_t, false>(pl, in, out, s);
+  return pl.rag ? launch_tiled_v<T, CONJ, uint32_t, true>(pl, in, out, s)
+                : launch_tiled_v<T, CONJ, uint32_t, false>(pl, in, out, s);
+}
+
+template <typename T, bool CONJ>
+int launch_copy_t(const void *in, void *out, uint64_t n, hipStream_t s, int cus)
+{
+  const uint64_t blocks = std::min<uint64_t>((n + kNT - 1) / kNT, (uint64_t)cus * 16);
+  if (blocks == 0) return 0;
+  hipLaunchKernelGGL((k_copy<T, CONJ>), dim3((unsigned)blocks), dim3(kNT), 0, s, static_cast<const T *>(in),
+                     static_cast<T *>(out), n);
+  return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
+}
+
+template <typename T, bool CONJ>
+int launch_scatter_t(const Fused &f, const void *in, void *out, hipStream_t s, int cus)
+{
+  ScatterParams h;
+  std::memset(&h, 0, sizeof(h));
+  h.rank = f.rank;
+  h.vol = 1;
+  std::vector<int> n2o(f.rank);
+  for (int i = 0; i < f.rank; ++i) n2o[f.o2n[i]] = i;
+  std::vector<uint64_t> ostr_out(f.rank);
+  uint64_t st = 1;
+  for (int q = 0; q < f.rank; ++q) { ostr_out[q] = st; st *= (uint64_t)f.ext[n2o[q]]; }
+  for (int i = 0; i < f.rank; ++i) {
+    h.ext[i] = (uint64_t)f.ext[i];
+    h.ostr[i] = ostr_out[f.o2n[i]];
+    h.vol *= h.ext[i];
+  }
+  const uint64_t blocks = std::min<uint64_t>((h.vol + kNT - 1) / kNT, (uint64_t)cus * 16);
+  hipLaunchKernelGGL((k_permute_scatter<T, CONJ>), dim3((unsigned)std::max<uint64_t>(blocks, 1)), dim3(kNT), 0, s,
+                     static_cast<const T *>(in), static_cast<T *>(out), h);
+  return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
+}
+
+template <typename T, bool CONJ>
+int dispatch(const Planned &pl, const void *in, void *out, uint64_t vol, hipStream_t s, int cus)
+{
+  if (pl.path == 0) return launch_copy_t<T, CONJ>(in, out, vol, s, cus);
+  if (pl.path == 1) return launch_tiled_t<T, CONJ>(pl, in, out, s);
+  return launch_scatter_t<T, CONJ>(pl.f, in, out, s, cus);
+}
+
+}  // namespace
+
+int plan_permute_info(int kind, int rank, const int *dims_in, const int *o2n, PermuteInfo *info)
+{
+  const int es = kind_size(kind);
+  if (es == 0 || rank < 0 || rank > kMaxRank) return TALSH_INVALID_ARGS;
+  Planned pl;
+  plan(es, rank, dims_in, o2n, pl, 256);
+  info->fused_rank = pl.f.rank;
+  info->path = pl.path;
+  info->tile_volume = pl.path == 1 ? (int)pl.tile.vol : 0;
+  info->in_run = pl.path == 1 ? pl.tile.in_run : 0;
+  info->out_run = pl.path == 1 ? pl.tile.out_run : 0;
+  info->grid = pl.grid;
+  return 0;
+}
+
+int launch_permute(const void *in, void *out, int kind, int rank, const int *dims_in, const int *o2n, bool conj,
+                   hipStream_t s)
+{
+  const int es = kind_size(kind);
+  if (es == 0 || rank < 0 || rank > kMaxRank) return TALSH_INVALID_ARGS;
+  std::vector<int> seen(rank, 0);
+  uint64_t vol = 1;
+  for (int i = 0; i < rank; ++i) {
+    if (dims_in[i] <= 0 || o2n[i] < 0 || o2n[i] >= rank || seen[o2n[i]]) return TALSH_INVALID_ARGS;
+    seen[o2n[i]] = 1;
+    vol *= (uint64_t)dims_in[i];
+  }
+  if (vol == 0) return 0;
+  if (in == out) return TALSH_INVALID_ARGS;  // out-of-place only
+  static int cus = num_cus_current();
+  Planned pl;
+  plan(es, rank, dims_in, o2n, pl, cus);
+  if (pl.path == 1) {
+    // the persistent grid and per-thread element slots must cover the tile
+    if ((long)pl.ept * kNT < (long)pl.tp.vol_tile || pl.grid <= 0) return TALSH_FAILURE;
+  }
+  const bool cj = conj && kind_complex(kind);
+  switch (kind) {
+  case R4: return dispatch<float, false>(pl, in, out, vol, s, cus);
+  case R8: return dispatch<double, false>(pl, in, out, vol, s, cus);
+  case C4: return cj ? dispatch<float2, true>(pl, in, out, vol, s, cus) : dispatch<float2, false>(pl, in, out, vol, s, cus);
+  case C8:
+    return cj ? dispatch<double2, true>(pl, in, out, vol, s, cus) : dispatch<double2, false>(pl, in, out, vol, s, cus);
+  default: return TALSH_INVALID_ARGS;
+  }
+}
+
+}  // namespace xt

@@ -1,0 +1,58 @@
+// Launchers of the engine's gfx950 kernels. All launches are asynchronous on
+// the given stream; every launcher checks its operand shapes against what the
+// kernel and grid assume before launching and returns 0 or a TAL-SH code.
+#pragma once
+
+#include "common.hpp"
+#include "runtime.hpp"
+
+namespace xt {
+
+// ---- DLF permutation (k_transpose.hip) ------------------------------------
+// out[idx'] = in[idx] where input dimension i becomes output dimension o2n[i]
+// (0-based); conj negates imaginary parts of complex kinds. Bit-exact.
+int launch_permute(const void *in, void *out, int kind, int rank, const int *dims_in, const int *o2n, bool conj,
+                   hipStream_t s);
+
+// Transpose plan summary for diagnostics and the plan selector.
+struct PermuteInfo {
+  int fused_rank;       // rank after dropping unit extents and fusing runs
+  int tile_volume;      // elements per LDS tile (0: direct copy path)
+  int in_run, out_run;  // contiguous run lengths (elements) on each side
+  int grid;             // workgroups launched
+  int path;             // 0 copy, 1 tiled, 2 scatter fallback
+};
+int plan_permute_info(int kind, int rank, const int *dims_in, const int *o2n, PermuteInfo *info);
+
+// ---- element-wise (k_elementwise.hip) -------------------------------------
+int launch_fill(void *d, int kind, size_t vol, double re, double im, hipStream_t s);
+int launch_scale(void *d, int kind, size_t vol, double re, double im, hipStream_t s);
+// d = (accumulate ? d : 0) + f * (conj ? conj(l) : l)
+int launch_axpy(void *d, const void *l, int kind, size_t vol, double re, double im, bool conj, bool accumulate,
+                hipStream_t s);
+// d[0] = (accumulate ? d[0] : 0) + alpha * sum_i l[i]*r[i] (conj flags per operand)
+int launch_dot(void *d, const void *l, const void *r, int kind, size_t n, double are, double aim, bool lconj,
+               bool rconj, bool accumulate, void *scratch, size_t scratch_bytes, hipStream_t s);
+
+// ---- matrix products (k_gemm.hip) -----------------------------------------
+// C(m x n, column-major, ldc = m) = beta*C + alpha * op(A) * op(B)
+//   a_kfast: A stored k x m with k fastest (else m x k with m fastest)
+//   b_kfast: B stored k x n with k fastest (else n x k with n fastest)
+// conj_a / conj_b conjugate complex operands; beta is 0 or 1.
+struct GemmArgs {
+  int kind;
+  R4Mode r4mode;
+  size_t m, n, k;
+  const void *a;
+  const void *b;
+  void *c;
+  bool a_kfast, b_kfast, conj_a, conj_b, beta_zero;
+  double alpha_re, alpha_im;
+  int tile;  // tile-shape choice (-1: default)
+};
+int launch_gemm(const GemmArgs &g, hipStream_t s);
+// Bytes of scratch needed by launch_gemm (BF16x3 split planes); 0 otherwise.
+size_t gemm_workspace_bytes(const GemmArgs &g);
+int launch_gemm_ws(const GemmArgs &g, void *ws, hipStream_t s);
+
+}  // namespace xt

@@ -1,0 +1,23 @@
+// Internal helpers shared by the C-ABI translation units.
+#pragma once
+
+#include "runtime.hpp"
+
+namespace xt {
+
+int canonical_dev(int flat);
+int resolve_device(int dev_id, int dev_kind);
+void *host_alloc(size_t bytes);
+void host_free(void *p);
+int image_alloc(int dev_flat, size_t bytes, talsh_dev_rsc_t *rsc);
+void image_release(const talsh_dev_rsc_t &r, Task *defer);
+void pending_free(const PendingFree &f);
+int tensor_add_image(talsh_tens_t *t, const talsh_dev_rsc_t &r, int kind);
+void tensor_remove_image(talsh_tens_t *t, int i, Task *defer);
+int find_image(const talsh_tens_t *t, int dev_flat, int kind);
+size_t tensor_volume(const talsh_tens_t *t);
+void image_wait(hipStream_t s, void *img);
+void image_touch(hipStream_t s, void *img);
+int prof_collect(int id, double *total_ms, long long *count);
+
+}  // namespace xt

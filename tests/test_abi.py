@@ -1,0 +1,150 @@
+"""The drop-in boundary: the engine library exports every entry point its
+headers declare, with the reference's struct layouts and constant values, and
+its host-side index logic (pattern parser, matricization planner) matches the
+reference's golden fixtures. CPU only."""
+import ctypes as C
+import gzip
+import json
+import os
+import re
+import subprocess
+
+import pytest
+
+from exatensor_amd import talsh as T
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INC = os.path.join(ROOT, "include")
+REF_INC = "/root/reference/TALSH"
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return T.load_library()
+
+
+def _declared_functions():
+    names = set()
+    for h in ("talsh.h", "tensor_algebra.h", "talsh_amd.h"):
+        text = open(os.path.join(INC, h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*\*?\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(",
+                             text, flags=re.M):
+            name = m.group(1)
+            if name not in ("if", "while", "for", "return", "sizeof", "typedef", "int"):
+                names.add(name)
+    return sorted(names)
+
+
+def test_every_declared_symbol_is_exported(lib):
+    names = _declared_functions()
+    assert len(names) > 150
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+_PROBE = r'''
+#include <cstdio>
+#include <cstddef>
+#include "talsh.h"
+#define S(t) printf(#t " %zu\n", sizeof(t));
+#define O(t, f) printf(#t "." #f " %zu\n", offsetof(t, f));
+#define V(c) printf(#c " %lld\n", (long long)(c));
+int main() {
+  S(talsh_tens_t) S(talsh_task_t) S(talsh_tens_op_t) S(talsh_tens_slice_t) S(talshTensArg_t)
+  S(talsh_tens_shape_t) S(talsh_tens_signature_t) S(talsh_tens_data_t) S(talsh_tens_dense_t) S(talsh_dev_rsc_t)
+  O(talsh_task_t, tens_args) O(talsh_task_t, data_vol) O(talsh_task_t, exec_time)
+  O(talsh_tens_op_t, symb_pattern) O(talsh_tens_op_t, tens_arg) O(talsh_tens_op_t, task_handle)
+  O(talsh_tens_op_t, exec_dev_id) O(talsh_tens_op_t, time_finished) O(talsh_dev_rsc_t, mem_attached)
+  V(TALSH_SUCCESS) V(TALSH_FAILURE) V(TALSH_NOT_AVAILABLE) V(TALSH_NOT_IMPLEMENTED) V(TALSH_NOT_INITIALIZED)
+  V(TALSH_INVALID_ARGS) V(TALSH_OBJECT_IS_EMPTY) V(TALSH_IN_PROGRESS) V(TALSH_INVALID_REQUEST)
+  V(TALSH_TASK_ERROR) V(TALSH_TASK_EMPTY) V(TALSH_TASK_COMPLETED) V(TALSH_TENSOR_CONTRACT) V(TALSH_OP_RETIRED)
+  V(TALSH_MAX_DEV_PRESENT) V(MAX_TENSOR_RANK) V(MAX_TENSOR_OPERANDS) V(R4) V(R8) V(C4) V(C8) V(NO_TYPE)
+  V(DEV_HOST) V(DEV_NVIDIA_GPU) V(DEV_INTEL_MIC) V(DEV_AMD_GPU) V(DEV_MAX) V(DEVICE_UNABLE) V(TRY_LATER)
+  V(NOT_CLEAN) V(COPY_MTT) V(COPY_KKK) V(COPY_TT) V(COPY_K) V(YEP) V(NOPE)
+  return 0;
+}
+'''
+
+
+def _probe(inc, tmp_path, tag):
+    src = tmp_path / f"probe_{tag}.cpp"
+    src.write_text(_PROBE)
+    exe = tmp_path / f"probe_{tag}"
+    subprocess.run(["g++", "-std=c++11", "-w", "-I", inc, str(src), "-o", str(exe)], check=True)
+    return subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+
+
+def test_struct_layouts_and_constants_match_reference(tmp_path):
+    ours = _probe(INC, tmp_path, "ours")
+    assert "talsh_tens_t 40" in ours
+    if not os.path.isdir(REF_INC):
+        pytest.skip("reference headers not present")
+    assert ours == _probe(REF_INC, tmp_path, "ref")
+
+
+def test_parser_matches_reference_fixtures(lib):
+    rows = json.load(open(os.path.join(ROOT, "tests", "golden", "patterns.json")))["rows"]
+    for row in rows:
+        e, dig, dr, lr, rr, cj = T.str2dig(lib, row["pattern"])
+        assert e == row["ierr"], row
+        if e == 0:
+            assert (dig, dr, lr, rr, cj) == (row["dig"], row["drank"], row["lrank"], row["rrank"], row["conj"]), row
+
+
+def test_permutation_planner_matches_reference_fixtures(lib):
+    with gzip.open(os.path.join(ROOT, "tests", "golden", "permutations.json.gz"), "rt") as f:
+        rows = json.load(f)["rows"]
+    for row in rows:
+        d, l, r = (C.c_int * 64)(), (C.c_int * 64)(), (C.c_int * 64)()
+        ncd, nlu, nru, nhu, ierr = (C.c_int() for _ in range(5))
+        dig = T.ints(row["dig"])
+        if row["hyper"]:
+            lib.get_contraction_permutations(row["tl"], row["tr"], row["lrank"], row["rrank"], dig, row["conj"], d, l,
+                                             r, C.byref(ncd), C.byref(nlu), C.byref(nru), C.byref(nhu), C.byref(ierr))
+        else:
+            lib.get_contr_permutations(row["tl"], row["tr"], row["lrank"], row["rrank"], dig, row["conj"], d, l, r,
+                                       C.byref(ncd), C.byref(nlu), C.byref(nru), C.byref(ierr))
+        assert ierr.value == row["ierr"], row
+        if ierr.value:
+            continue
+        assert (ncd.value, nlu.value, nru.value) == (row["ncd"], row["nlu"], row["nru"]), row
+        if row["hyper"]:
+            assert nhu.value == row["nhu"], row
+        assert list(l[1:row["lrank"] + 1]) == row["lprm"][1:], row
+        assert list(r[1:row["rrank"] + 1]) == row["rprm"][1:], row
+        assert list(d[1:len(row["dprm"])]) == row["dprm"][1:], row
+
+
+@pytest.fixture(scope="module")
+def host_engine(lib):
+    eng = T.Engine(gpus=[])
+    yield eng
+    eng.shutdown()
+
+
+def test_host_only_engine_semantics(lib, host_engine):
+    t = T.Tensor(lib, T.R8, [3, 4, 5], 0, 2.5)
+    assert t.err == 0
+    assert lib.talshTensorRank(t.ptr) == 3 and lib.talshTensorVolume(t.ptr) == 60
+    assert (t.download() == 2.5).all()
+    # a second construct on a non-empty tensor is refused
+    assert lib.talshTensorConstruct(t.ptr, T.R8, 1, T.ints([2]), 0, None, -1, None, 0.0, 0.0) == \
+        T.TALSH_OBJECT_NOT_EMPTY
+    # tensor operations execute on GPUs only
+    assert lib.talshTensorInit(t.ptr, 1.0, 0.0, 0, T.DEV_HOST, T.COPY_M, None) == T.TALSH_NOT_AVAILABLE
+    assert lib.talshTensorContract(b"D(a,b,c)+=L(a,b,c)*R()", t.ptr, t.ptr, t.ptr, 1.0, 0.0, T.DEV_DEFAULT,
+                                   T.DEV_DEFAULT, T.COPY_MTT, T.YEP, None) == T.TALSH_NOT_AVAILABLE
+    task = T.TalshTask()
+    lib.talshTaskClean(C.byref(task))
+    assert lib.talshTaskStatus(C.byref(task)) == T.TALSH_TASK_EMPTY
+    assert t.destruct() == 0
+    assert lib.talshTensorIsEmpty(t.ptr) == T.YEP
+
+
+def test_device_id_codec(lib):
+    assert lib.talshFlatDevId(T.DEV_HOST, 0) == 0
+    assert lib.talshFlatDevId(T.DEV_NVIDIA_GPU, 0) == 1
+    assert lib.talshFlatDevId(T.DEV_AMD_GPU, 0) == 17
+    k = C.c_int()
+    assert lib.talshKindDevId(18, C.byref(k)) == 1 and k.value == T.DEV_AMD_GPU
