@@ -973,7 +973,8 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
   struct Body {
     Arg *D, *L, *R; ContractPlan P; std::vector<int> dig, dd, ld, rd; int lr, rr; bool lconj, rconj, acc;
     double are, aim; int tile;
-    void *lt = nullptr, *rt = nullptr, *dt = nullptr, *scr = nullptr, *tt = nullptr;
+    void *lt = nullptr, *rt = nullptr, *dt = nullptr, *scr = nullptr, *tt = nullptr, *ws = nullptr;
+    size_t wsb = 0;
     int allocate(Temps &t)
     {
       const size_t es = kind_size(D->kind);
@@ -999,6 +1000,12 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
       if (!P.l_o2n.empty()) e = t.alloc(tensor_volume(L->t) * es, &lt);
       if (!e && !P.r_o2n.empty()) e = t.alloc(tensor_volume(R->t) * es, &rt);
       if (!e && !P.d_in_o2n.empty()) e = t.alloc(tensor_volume(D->t) * es, &dt);
+      if (!e) {
+        GemmArgs g{};
+        g.kind = D->kind; g.r4mode = t.d->r4mode; g.m = P.m; g.n = P.n; g.k = P.k;
+        wsb = gemm_workspace_bytes(g);
+        if (wsb) e = t.alloc(wsb, &ws);
+      }
       return e;
     }
     int enqueue(Task *k, Device *dev)
@@ -1079,7 +1086,7 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
       g.c = dc;
       (void)hipEventRecord(k->ev_mm0, s);
       prof_begin(s, kKGemm, pr);
-      int e = launch_gemm(g, s);
+      int e = launch_gemm_ws(g, ws, s);
       prof_end(s, kKGemm, pr);
       (void)hipEventRecord(k->ev_mm1, s);
       k->has_mm = true;
@@ -1172,6 +1179,50 @@ int talshAmdPlanCandidates(const char *cptrn, int drank, const int *ddims, int l
   for (int i = 0; i < (int)all.size() && i < cap; ++i)
     if (moved) moved[i] = all[i].bytes_moved;
   return 0;
+}
+
+int talshAmdPermutePlan(int data_kind, int rank, const int *dims, const int *o2n, int *info)
+{
+  if (!dims || !o2n || !info) return TALSH_INVALID_ARGS;
+  PermuteInfo pi{};
+  const int e = plan_permute_info(data_kind, rank, dims, o2n, &pi);
+  if (e) return e;
+  info[0] = pi.fused_rank; info[1] = pi.path; info[2] = pi.tile_volume;
+  info[3] = pi.in_run; info[4] = pi.out_run; info[5] = pi.grid;
+  return 0;
+}
+
+int talshAmdGemm(int data_kind, int fast_math, size_t m, size_t n, size_t k, const void *a, int a_kfast,
+                 const void *b, int b_kfast, void *c, double alpha_re, double alpha_im, int beta_zero, int tile,
+                 void *workspace, size_t workspace_bytes)
+{
+  Engine &E = engine();
+  if (!E.on || E.gpus.empty()) return TALSH_NOT_INITIALIZED;
+  Device *d = E.gpus[0];
+  GemmArgs g{};
+  g.kind = data_kind;
+  g.r4mode = fast_math ? R4Mode::kBF16x3 : R4Mode::kNativeF32;
+  g.m = m; g.n = n; g.k = k; g.a = a; g.b = b; g.c = c;
+  g.a_kfast = a_kfast; g.b_kfast = b_kfast; g.beta_zero = beta_zero;
+  g.alpha_re = alpha_re; g.alpha_im = alpha_im; g.tile = tile;
+  if (gemm_workspace_bytes(g) > workspace_bytes) return TALSH_INVALID_ARGS;
+  (void)hipSetDevice(d->ordinal);
+  hipStream_t s = d->streams[0];
+  hipEvent_t pr[2];
+  prof_begin(s, kKGemm, pr);
+  const int e = launch_gemm_ws(g, workspace, s);
+  prof_end(s, kKGemm, pr);
+  if (e) return e;
+  return XT_HIP_OK(hipStreamSynchronize(s)) ? 0 : TALSH_FAILURE;
+}
+
+size_t talshAmdGemmWorkspace(int data_kind, int fast_math, size_t m, size_t n, size_t k)
+{
+  GemmArgs g{};
+  g.kind = data_kind;
+  g.r4mode = fast_math ? R4Mode::kBF16x3 : R4Mode::kNativeF32;
+  g.m = m; g.n = n; g.k = k;
+  return gemm_workspace_bytes(g);
 }
 
 int talshAmdDeviceSynchronize(int dev_num)

@@ -18,12 +18,14 @@ namespace {
 
 constexpr int kNT = 256;
 
-__device__ __forceinline__ float mul(float a, float b) { return __fmul_rn(a, b); }
-__device__ __forceinline__ double mul(double a, double b) { return __dmul_rn(a, b); }
-__device__ __forceinline__ float add(float a, float b) { return __fadd_rn(a, b); }
-__device__ __forceinline__ double add(double a, double b) { return __dadd_rn(a, b); }
-__device__ __forceinline__ float sub(float a, float b) { return __fsub_rn(a, b); }
-__device__ __forceinline__ double sub(double a, double b) { return __dsub_rn(a, b); }
+// (written as plain operators under the no-contraction pragma above: the
+// library's __fmul_rn-style helpers are defined before it and may be fused)
+__device__ __forceinline__ float mul(float a, float b) { return a * b; }
+__device__ __forceinline__ double mul(double a, double b) { return a * b; }
+__device__ __forceinline__ float add(float a, float b) { return a + b; }
+__device__ __forceinline__ double add(double a, double b) { return a + b; }
+__device__ __forceinline__ float sub(float a, float b) { return a - b; }
+__device__ __forceinline__ double sub(double a, double b) { return a - b; }
 
 template <typename R>
 struct Cx { R re, im; };
@@ -208,7 +210,7 @@ int launch_fill(void *d, int kind, size_t vol, double re, double im, hipStream_t
   const int g = grid_for(vol);
   switch (kind) {
   case R4: hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kNT), 0, s, (float *)d, (uint64_t)vol, (float)real_factor(re, im)); break;
-  case R8: hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kNT), 0, s, (double *)d, (uint64_t)vol, re); break;
+  case R8: hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kNT), 0, s, (double *)d, (uint64_t)vol, real_factor(re, im)); break;
   case C4: hipLaunchKernelGGL(k_fill<float2>, dim3(g), dim3(kNT), 0, s, (float2 *)d, (uint64_t)vol, make_float2((float)re, (float)im)); break;
   case C8: hipLaunchKernelGGL(k_fill<double2>, dim3(g), dim3(kNT), 0, s, (double2 *)d, (uint64_t)vol, make_double2(re, im)); break;
   default: return TALSH_INVALID_ARGS;

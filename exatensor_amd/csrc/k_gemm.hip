@@ -74,8 +74,8 @@ __device__ __forceinline__ void tile_of(const GemmParams &p, int &tm, int &tn)
 // ---------------------------------------------------------------------------
 // FP64 / FP32 16x16x4 family. Workgroup: 256 threads = 2x2 waves; each wave
 // owns a (BM/2) x (BN/2) sub-tile of 16x16 MFMA blocks.
-template <typename T, bool CPLX, bool AK, bool BKF, int BM, int BN, int BK>
-__global__ __launch_bounds__(256) void k_gemm_mfma(const GemmParams p)
+template <typename T, bool CPLX, bool AK, bool BKF, int BM, int BN, int BK, int OCC>
+__global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
 {
   constexpr int NP = CPLX ? 2 : 1;             // LDS planes (re, im)
   constexpr int PADM = 4, PADN = 4;
@@ -263,56 +263,98 @@ __device__ __forceinline__ f4 mfma_bf(const us8 &a, const us8 &b, f4 c)
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
 }
 
-template <bool AK, bool BKF, int BM, int BN>
-__global__ __launch_bounds__(256) void k_gemm_bf16x3(const GemmParams p)
+// Split pass: FP32 operand (k-fast rows x K, or row-fast) -> three bf16 planes
+// stored k-fast, zero-padded to rows_pad x k_pad (k_pad a multiple of 32), so
+// the matrix kernel needs no bounds checks on loads.
+template <bool KFAST>
+__global__ __launch_bounds__(256) void k_split3(const float *__restrict__ x, int64_t rows, int64_t k, int64_t ld,
+                                                 unsigned short *__restrict__ planes, int64_t rows_pad, int64_t k_pad)
 {
-  constexpr int BK = 32;                       // one MFMA k-step per LDS tile
-  constexpr int KP = BK + 8;                   // padded k extent of an LDS row (bf16)
+  const int64_t kc_n = k_pad / 8;
+  const int64_t total = rows_pad * kc_n;
+  const int64_t plane = rows_pad * k_pad;
+  const int64_t st = (int64_t)gridDim.x * 256;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += st) {
+    int64_t r, kc;
+    if (KFAST) { kc = idx % kc_n; r = idx / kc_n; }
+    else { r = idx % rows_pad; kc = idx / rows_pad; }  // consecutive threads read consecutive rows
+    us8 h0, h1, h2;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int64_t kk = kc * 8 + i;
+      float v = 0.f;
+      if (r < rows && kk < k) v = KFAST ? x[r * ld + kk] : x[kk * ld + r];
+      unsigned short a, b, c;
+      split3(v, a, b, c);
+      h0[i] = a; h1[i] = b; h2[i] = c;
+    }
+    const int64_t off = r * k_pad + kc * 8;
+    *reinterpret_cast<us8 *>(planes + off) = h0;
+    *reinterpret_cast<us8 *>(planes + plane + off) = h1;
+    *reinterpret_cast<us8 *>(planes + 2 * plane + off) = h2;
+  }
+}
+
+struct B3Params {
+  const unsigned short *a3;  // 3 planes, rows_pad_a x k_pad each
+  const unsigned short *b3;
+  float *c;
+  int64_t m, n, k_pad, a_rows_pad, b_rows_pad, ldc;
+  double alpha;
+  int beta_zero;
+  int tiles_m, tiles_n;
+};
+
+__device__ __forceinline__ void tile_of_b3(const B3Params &p, int &tm, int &tn)
+{
+  GemmParams q{};
+  q.tiles_m = p.tiles_m;
+  q.tiles_n = p.tiles_n;
+  q.xcd_remap = 1;
+  tile_of(q, tm, tn);
+}
+
+// Matrix kernel of the three-way split: six bf16 products per k-step, FP32
+// MFMA accumulation folded into FP64 every kFold stages.
+template <int BM, int BN>
+__global__ __launch_bounds__(256) void k_gemm_bf16x3(const B3Params p)
+{
+  constexpr int BK = 32, KP = BK + 8;
   constexpr int FM = BM / 32, FN = BN / 32;
-  constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
-  constexpr int kFold = 8;                     // k-tiles between FP64 folds
+  constexpr int CA = 3 * BM * (BK / 8) / 256;   // 16-byte chunks of A per thread per stage
+  constexpr int CB = 3 * BN * (BK / 8) / 256;
+  constexpr int kFold = 8;
   __shared__ __align__(16) unsigned short As[2][3][BM][KP];
   __shared__ __align__(16) unsigned short Bs[2][3][BN][KP];
-
   int tm, tn;
-  tile_of(p, tm, tn);
+  tile_of_b3(p, tm, tn);
   const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * (BN / 2);
-  const float *A = static_cast<const float *>(p.a);
-  const float *B = static_cast<const float *>(p.b);
-  int a_r, a_k, b_r, b_k;
-  if (AK) { a_r = tid / (BK / LA); a_k = (tid % (BK / LA)) * LA; }
-  else { a_k = tid / (BM / LA); a_r = (tid % (BM / LA)) * LA; }
-  if (BKF) { b_r = tid / (BK / LB); b_k = (tid % (BK / LB)) * LB; }
-  else { b_k = tid / (BN / LB); b_r = (tid % (BN / LB)) * LB; }
-  float ra[LA], rb[LB];
+  const int64_t pa = p.a_rows_pad * p.k_pad, pb = p.b_rows_pad * p.k_pad;
+  us8 ra[CA], rb[CB];
   auto load = [&](int64_t k0) {
 #pragma unroll
-    for (int i = 0; i < LA; ++i) {
-      const int64_t r = m0 + a_r + (AK ? 0 : i), kk = k0 + a_k + (AK ? i : 0);
-      ra[i] = (r < p.m && kk < p.k) ? A[AK ? (r * p.lda + kk) : (kk * p.lda + r)] : 0.f;
+    for (int i = 0; i < CA; ++i) {
+      const int c = tid + i * 256, q = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
+      ra[i] = *reinterpret_cast<const us8 *>(p.a3 + q * pa + (m0 + r) * p.k_pad + k0 + kc * 8);
     }
 #pragma unroll
-    for (int i = 0; i < LB; ++i) {
-      const int64_t r = n0 + b_r + (BKF ? 0 : i), kk = k0 + b_k + (BKF ? i : 0);
-      rb[i] = (r < p.n && kk < p.k) ? B[BKF ? (r * p.ldb + kk) : (kk * p.ldb + r)] : 0.f;
+    for (int i = 0; i < CB; ++i) {
+      const int c = tid + i * 256, q = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
+      rb[i] = *reinterpret_cast<const us8 *>(p.b3 + q * pb + (n0 + r) * p.k_pad + k0 + kc * 8);
     }
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < LA; ++i) {
-      unsigned short h0, h1, h2;
-      split3(ra[i], h0, h1, h2);
-      const int r = a_r + (AK ? 0 : i), kk = a_k + (AK ? i : 0);
-      As[buf][0][r][kk] = h0; As[buf][1][r][kk] = h1; As[buf][2][r][kk] = h2;
+    for (int i = 0; i < CA; ++i) {
+      const int c = tid + i * 256, q = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
+      *reinterpret_cast<us8 *>(&As[buf][q][r][kc * 8]) = ra[i];
     }
 #pragma unroll
-    for (int i = 0; i < LB; ++i) {
-      unsigned short h0, h1, h2;
-      split3(rb[i], h0, h1, h2);
-      const int r = b_r + (BKF ? 0 : i), kk = b_k + (BKF ? i : 0);
-      Bs[buf][0][r][kk] = h0; Bs[buf][1][r][kk] = h1; Bs[buf][2][r][kk] = h2;
+    for (int i = 0; i < CB; ++i) {
+      const int c = tid + i * 256, q = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
+      *reinterpret_cast<us8 *>(&Bs[buf][q][r][kc * 8]) = rb[i];
     }
   };
   f4 acc[FM][FN];
@@ -325,7 +367,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const GemmParams p)
 #pragma unroll
       for (int r = 0; r < 4; ++r) sum[i][j][r] = 0.0;
     }
-  const int64_t nk = (p.k + BK - 1) / BK;
+  const int64_t nk = p.k_pad / BK;
   load(0);
   store(0);
   __syncthreads();
@@ -345,7 +387,6 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const GemmParams p)
     for (int i = 0; i < FM; ++i) {
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
-        // smallest terms first
         acc[i][j] = mfma_bf(fa[0][i], fb[2][j], acc[i][j]);
         acc[i][j] = mfma_bf(fa[1][i], fb[1][j], acc[i][j]);
         acc[i][j] = mfma_bf(fa[2][i], fb[0][j], acc[i][j]);
@@ -367,7 +408,6 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const GemmParams p)
     if (kt + 1 < nk) store(cur ^ 1);
     __syncthreads();
   }
-  float *C = static_cast<float *>(p.c);
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -378,9 +418,9 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const GemmParams p)
         const int64_t row = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
         if (row < p.m && col < p.n) {
           const int64_t off = row + col * p.ldc;
-          double v = p.alpha_re * sum[i][j][r];
-          if (!p.beta_zero) v += (double)C[off];
-          C[off] = (float)v;
+          double v = p.alpha * sum[i][j][r];
+          if (!p.beta_zero) v += (double)p.c[off];
+          p.c[off] = (float)v;
         }
       }
     }
@@ -394,37 +434,66 @@ int launch_grid(K kern, const GemmParams &p, hipStream_t s)
   return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
 }
 
-template <typename T, bool CPLX, int BM, int BN, int BK>
+template <typename T, bool CPLX, int BM, int BN, int BK, int OCC>
 int launch_family(GemmParams p, bool ak, bool bk, hipStream_t s)
 {
   p.tiles_m = (int)((p.m + BM - 1) / BM);
   p.tiles_n = (int)((p.n + BN - 1) / BN);
-  if (ak && bk) return launch_grid(k_gemm_mfma<T, CPLX, true, true, BM, BN, BK>, p, s);
-  if (ak) return launch_grid(k_gemm_mfma<T, CPLX, true, false, BM, BN, BK>, p, s);
-  if (bk) return launch_grid(k_gemm_mfma<T, CPLX, false, true, BM, BN, BK>, p, s);
-  return launch_grid(k_gemm_mfma<T, CPLX, false, false, BM, BN, BK>, p, s);
+  if (ak && bk) return launch_grid(k_gemm_mfma<T, CPLX, true, true, BM, BN, BK, OCC>, p, s);
+  if (ak) return launch_grid(k_gemm_mfma<T, CPLX, true, false, BM, BN, BK, OCC>, p, s);
+  if (bk) return launch_grid(k_gemm_mfma<T, CPLX, false, true, BM, BN, BK, OCC>, p, s);
+  return launch_grid(k_gemm_mfma<T, CPLX, false, false, BM, BN, BK, OCC>, p, s);
 }
 
-template <int BM, int BN>
-int launch_bf16x3(GemmParams p, bool ak, bool bk, hipStream_t s)
+constexpr int kB3Tile = 128;
+
+int64_t round_up(int64_t x, int64_t q) { return (x + q - 1) / q * q; }
+
+int launch_bf16x3(const GemmArgs &g, void *ws, hipStream_t s)
 {
-  p.tiles_m = (int)((p.m + BM - 1) / BM);
-  p.tiles_n = (int)((p.n + BN - 1) / BN);
-  if (ak && bk) return launch_grid(k_gemm_bf16x3<true, true, BM, BN>, p, s);
-  if (ak) return launch_grid(k_gemm_bf16x3<true, false, BM, BN>, p, s);
-  if (bk) return launch_grid(k_gemm_bf16x3<false, true, BM, BN>, p, s);
-  return launch_grid(k_gemm_bf16x3<false, false, BM, BN>, p, s);
+  const int64_t kp = round_up(std::max<int64_t>((int64_t)g.k, 1), 32);
+  const int64_t mp = round_up((int64_t)g.m, kB3Tile), np = round_up((int64_t)g.n, kB3Tile);
+  if (!ws) return TALSH_INVALID_ARGS;
+  unsigned short *a3 = static_cast<unsigned short *>(ws);
+  unsigned short *b3 = a3 + 3 * mp * kp;
+  auto split = [&](const void *x, int64_t rows, bool kfast, unsigned short *out, int64_t rows_pad) {
+    const int64_t total = rows_pad * (kp / 8);
+    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
+    const int64_t ld = kfast ? (int64_t)g.k : rows;
+    if (kfast)
+      hipLaunchKernelGGL(k_split3<true>, dim3(blocks), dim3(256), 0, s, (const float *)x, rows, (int64_t)g.k, ld, out,
+                         rows_pad, kp);
+    else
+      hipLaunchKernelGGL(k_split3<false>, dim3(blocks), dim3(256), 0, s, (const float *)x, rows, (int64_t)g.k, ld, out,
+                         rows_pad, kp);
+  };
+  split(g.a, (int64_t)g.m, g.a_kfast, a3, mp);
+  split(g.b, (int64_t)g.n, g.b_kfast, b3, np);
+  B3Params p{};
+  p.a3 = a3; p.b3 = b3; p.c = static_cast<float *>(g.c);
+  p.m = (int64_t)g.m; p.n = (int64_t)g.n; p.k_pad = kp; p.a_rows_pad = mp; p.b_rows_pad = np; p.ldc = (int64_t)g.m;
+  p.alpha = g.alpha_re; p.beta_zero = g.beta_zero;
+  p.tiles_m = (int)(mp / kB3Tile); p.tiles_n = (int)(np / kB3Tile);
+  hipLaunchKernelGGL((k_gemm_bf16x3<kB3Tile, kB3Tile>), dim3(p.tiles_m * p.tiles_n), dim3(256), 0, s, p);
+  return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
 }
 
 }  // namespace
 
-size_t gemm_workspace_bytes(const GemmArgs &) { return 0; }
+size_t gemm_workspace_bytes(const GemmArgs &g)
+{
+  if (g.kind != R4 || g.r4mode != R4Mode::kBF16x3) return 0;
+  const int64_t kp = round_up(std::max<int64_t>((int64_t)g.k, 1), 32);
+  const int64_t mp = round_up((int64_t)g.m, kB3Tile), np = round_up((int64_t)g.n, kB3Tile);
+  return (size_t)(3 * (mp + np) * kp) * sizeof(unsigned short);
+}
 
-int launch_gemm_ws(const GemmArgs &g, void *, hipStream_t s) { return launch_gemm(g, s); }
+int launch_gemm(const GemmArgs &g, hipStream_t s) { return launch_gemm_ws(g, nullptr, s); }
 
-int launch_gemm(const GemmArgs &g, hipStream_t s)
+int launch_gemm_ws(const GemmArgs &g, void *ws, hipStream_t s)
 {
   if (g.m == 0 || g.n == 0) return 0;
+  if (g.k == 0 && !g.beta_zero) return 0;  // empty contraction leaves C unchanged
   GemmParams p{};
   p.a = g.a; p.b = g.b; p.c = g.c;
   p.m = (int64_t)g.m; p.n = (int64_t)g.n; p.k = (int64_t)g.k;
@@ -435,19 +504,24 @@ int launch_gemm(const GemmArgs &g, hipStream_t s)
   p.beta_zero = g.beta_zero ? 1 : 0;
   p.conj_a = g.conj_a; p.conj_b = g.conj_b;
   p.xcd_remap = 1;
-  // tile counts must fit the 1-D grid
   if ((p.m + 63) / 64 * ((p.n + 63) / 64) >= (1ll << 31)) return TALSH_LIMIT_EXCEEDED;
-  if (g.k == 0) {
-    // empty contraction: C = beta * C
-    if (!g.beta_zero) return 0;
-  }
+  const int tile = g.tile < 0 ? 0 : g.tile;
   switch (g.kind) {
-  case R8: return launch_family<double, false, 128, 128, 16>(p, g.a_kfast, g.b_kfast, s);
-  case C8: return launch_family<double, true, 64, 64, 16>(p, g.a_kfast, g.b_kfast, s);
+  case R8:
+    switch (tile) {
+    case 1: return launch_family<double, false, 128, 128, 16, 1>(p, g.a_kfast, g.b_kfast, s);
+    case 2: return launch_family<double, false, 64, 64, 16, 2>(p, g.a_kfast, g.b_kfast, s);
+    default: return launch_family<double, false, 128, 128, 16, 2>(p, g.a_kfast, g.b_kfast, s);
+    }
+  case C8: return launch_family<double, true, 64, 64, 16, 2>(p, g.a_kfast, g.b_kfast, s);
   case R4:
-    if (g.r4mode == R4Mode::kBF16x3) return launch_bf16x3<128, 128>(p, g.a_kfast, g.b_kfast, s);
-    return launch_family<float, false, 128, 128, 16>(p, g.a_kfast, g.b_kfast, s);
-  case C4: return launch_family<float, true, 64, 64, 16>(p, g.a_kfast, g.b_kfast, s);
+    if (g.r4mode == R4Mode::kBF16x3) return launch_bf16x3(g, ws, s);
+    switch (tile) {
+    case 1: return launch_family<float, false, 128, 128, 16, 1>(p, g.a_kfast, g.b_kfast, s);
+    case 2: return launch_family<float, false, 64, 64, 16, 2>(p, g.a_kfast, g.b_kfast, s);
+    default: return launch_family<float, false, 128, 128, 16, 2>(p, g.a_kfast, g.b_kfast, s);
+    }
+  case C4: return launch_family<float, true, 64, 64, 16, 2>(p, g.a_kfast, g.b_kfast, s);
   default: return TALSH_INVALID_ARGS;
   }
 }

@@ -295,6 +295,56 @@ TilePlan choose_tile(const Fused &f, int target_in, int target_out)
   return tp;
 }
 
+// Grows a tile that meets its run targets but is too small to keep a
+// workgroup busy (e.g. when the input- and output-minor dimensions coincide):
+// alternately extends the input-order and output-order prefixes (doubling the
+// first partial dimension, or opening the next one) until vol >= min_vol.
+void grow_tile(const Fused &f, TilePlan &tp, long min_vol, long max_vol)
+{
+  const int r = f.rank;
+  std::vector<int> n2o(r);
+  for (int i = 0; i < r; ++i) n2o[f.o2n[i]] = i;
+  auto volume = [&]() {
+    long v = 1;
+    for (int i = 0; i < r; ++i)
+      if (tp.t[i]) v *= tp.t[i];
+    return v;
+  };
+  int side = 0, stuck = 0;
+  while (volume() < min_vol && stuck < 2) {
+    int dim = -1;
+    for (int q = 0; q < r; ++q) {
+      const int i = side == 0 ? q : n2o[q];
+      if (tp.t[i] < f.ext[i]) { dim = i; break; }
+    }
+    side ^= 1;
+    if (dim < 0) { ++stuck; continue; }
+    const int old = tp.t[dim];
+    const int want = old == 0 ? 2 : old * 2;
+    const int nt = pick_split(f.ext[dim], std::min(want, f.ext[dim]));
+    const long nv = volume() / std::max(old, 1) * nt;
+    if (nv > max_vol) { ++stuck; continue; }
+    stuck = 0;
+    tp.t[dim] = nt;
+  }
+  tp.vol = volume();
+  long run = 1;
+  for (int i = 0; i < r; ++i) {
+    if (tp.t[i] == 0) break;
+    run *= tp.t[i];
+    if (tp.t[i] < f.ext[i]) break;
+  }
+  tp.in_run = (int)std::min<long>(run, 1 << 30);
+  run = 1;
+  for (int q = 0; q < r; ++q) {
+    const int i = n2o[q];
+    if (tp.t[i] == 0) break;
+    run *= tp.t[i];
+    if (tp.t[i] < f.ext[i]) break;
+  }
+  tp.out_run = (int)std::min<long>(run, 1 << 30);
+}
+
 // LDS bank-conflict degree of the store phase for a padding period.
 int conflict_cost(const std::vector<uint32_t> &slots, int words_per_elem, uint32_t period)
 {
@@ -367,6 +417,15 @@ int plan(int es, int rank, const int *dims, const int *o2n, Planned &pl, int cus
     for (int x : tp.t) ntd += x > 0;
     if ((tp.vol * es <= max_tile_bytes && tp.vol <= (long)kNT * kMaxEPT && ntd <= kMaxTileDims) || target <= 1) break;
     target = std::max(1, target / 2);
+  }
+  {
+    const long max_vol = std::min<long>((long)kNT * kMaxEPT, max_tile_bytes / es);
+    const long min_vol = std::min<long>(2048, max_vol);
+    TilePlan g = tp;
+    grow_tile(f, g, min_vol, max_vol);
+    int gd = 0;
+    for (int x : g.t) gd += x > 0;
+    if (gd <= kMaxTileDims) tp = g;
   }
   int ntd = 0, nrag = 0;
   for (int i = 0; i < f.rank; ++i) {

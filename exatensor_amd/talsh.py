@@ -124,6 +124,11 @@ def load_library(path=LIB_PATH):
         "talshAmdPlanCandidates": (C.c_int, [C.c_char_p, C.c_int, _IP, C.c_int, _IP, C.c_int, _IP, C.c_int, C.c_int,
                                              C.c_int, C.POINTER(C.c_double), _IP]),
         "talshAmdDeviceSynchronize": (C.c_int, [C.c_int]),
+        "talshAmdPermutePlan": (C.c_int, [C.c_int, C.c_int, _IP, _IP, _IP]),
+        "talshAmdGemm": (C.c_int, [C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p, C.c_int,
+                                   C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int,
+                                   C.c_void_p, C.c_size_t]),
+        "talshAmdGemmWorkspace": (C.c_size_t, [C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -35,6 +35,23 @@ int talshAmdSetPlanVariant(int variant, int tile);
 int talshAmdPlanCandidates(const char *cptrn, int drank, const int *ddims, int lrank, const int *ldims, int rrank,
                            const int *rdims, int data_kind, int accumulative, int cap, double *moved, int *ncand);
 
+/* Plan of the DLF permutation kernel for a tensor of data_kind with extents
+ * dims[rank] and 0-based O2N permutation o2n[rank]: info[0] fused rank,
+ * info[1] path (0 copy, 1 LDS-tiled, 2 scatter), info[2] tile volume,
+ * info[3] input run, info[4] output run (elements), info[5] workgroups. */
+int talshAmdPermutePlan(int data_kind, int rank, const int *dims, const int *o2n, int *info);
+
+/* Matrix product on GPU 0 with the engine's kernels (device pointers):
+ * C(m x n, column-major) = alpha * op(A) * op(B) (+ C unless beta_zero);
+ * A is k x m k-fastest (a_kfast) or m x k m-fastest, B is k x n k-fastest
+ * (b_kfast) or n x k n-fastest. fast_math selects the three-way BF16 split
+ * for R4, which needs talshAmdGemmWorkspace() bytes of device workspace.
+ * Synchronous; used by the autotuner and the tests. */
+int talshAmdGemm(int data_kind, int fast_math, size_t m, size_t n, size_t k, const void *a, int a_kfast,
+                 const void *b, int b_kfast, void *c, double alpha_re, double alpha_im, int beta_zero, int tile,
+                 void *workspace, size_t workspace_bytes);
+size_t talshAmdGemmWorkspace(int data_kind, int fast_math, size_t m, size_t n, size_t k);
+
 /* Synchronize all engine streams of a GPU (kind-specific number). */
 int talshAmdDeviceSynchronize(int dev_num);
 
