@@ -1188,7 +1188,7 @@ int talshAmdPermutePlan(int data_kind, int rank, const int *dims, const int *o2n
   const int e = plan_permute_info(data_kind, rank, dims, o2n, &pi);
   if (e) return e;
   info[0] = pi.fused_rank; info[1] = pi.path; info[2] = pi.tile_volume;
-  info[3] = pi.in_run; info[4] = pi.out_run; info[5] = pi.grid;
+  info[3] = pi.in_run; info[4] = pi.out_run; info[5] = pi.grid; info[6] = pi.vec;
   return 0;
 }
 

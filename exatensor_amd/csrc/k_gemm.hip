@@ -50,6 +50,7 @@ struct GemmParams {
   int conj_a, conj_b;
   int tiles_m, tiles_n;
   int xcd_remap;             // 1: blockIdx -> tile remap that keeps neighbours on one XCD
+  int vec_a, vec_b;          // 16-byte aligned rows of k-fast operands (vector loads allowed)
 };
 
 // Maps a workgroup to an output tile: first gather the workgroups that the
@@ -74,17 +75,30 @@ __device__ __forceinline__ void tile_of(const GemmParams &p, int &tm, int &tn)
 // ---------------------------------------------------------------------------
 // FP64 / FP32 16x16x4 family. Workgroup: 256 threads = 2x2 waves; each wave
 // owns a (BM/2) x (BN/2) sub-tile of 16x16 MFMA blocks.
+//
+// Operand tiles sit in LDS with k fastest ([row][k], padded rows). Inside a
+// 16-deep k slice, lane group g = lane/16 feeds k = 4g+s at MFMA step s, so a
+// lane's four steps use four consecutive k and its fragment for the whole
+// slice is one contiguous LDS read (summation order over k is immaterial).
+template <typename T>
+struct Frag4;
+template <>
+struct Frag4<float> { typedef float4 type; };
+template <>
+struct Frag4<double> { typedef double4 type; };
+
 template <typename T, bool CPLX, bool AK, bool BKF, int BM, int BN, int BK, int OCC>
 __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
 {
-  constexpr int NP = CPLX ? 2 : 1;             // LDS planes (re, im)
-  constexpr int PADM = 4, PADN = 4;
-  constexpr int FM = BM / 32, FN = BN / 32;    // MFMA blocks per wave along m / n
-  constexpr int LA = BM * BK / 256;            // A elements staged per thread
+  constexpr int NP = CPLX ? 2 : 1;                 // LDS planes (re, im)
+  constexpr int KP = BK + (sizeof(T) == 4 ? 4 : 2); // padded k extent of an LDS row
+  constexpr int FM = BM / 32, FN = BN / 32;        // MFMA blocks per wave along m / n
+  constexpr int LA = BM * BK / 256;                // A elements staged per thread
   constexpr int LB = BN * BK / 256;
   typedef typename Acc<T>::type AccT;
-  __shared__ T As[2][NP][BK][BM + PADM];
-  __shared__ T Bs[2][NP][BK][BN + PADN];
+  typedef typename Frag4<T>::type F4;
+  __shared__ __align__(16) T As[2][NP][BM][KP];
+  __shared__ __align__(16) T Bs[2][NP][BN][KP];
 
   int tm, tn;
   tile_of(p, tm, tn);
@@ -93,9 +107,10 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
   const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * (BN / 2);
   const T *A = static_cast<const T *>(p.a);
   const T *B = static_cast<const T *>(p.b);
+  const bool interior_mn = (m0 + BM <= p.m) && (n0 + BN <= p.n);
 
-  // staging coordinates: for k-fast operands a thread covers LA consecutive k
-  // of one row; for row-fast operands LA consecutive rows of one k.
+  // staging coordinates: k-fast operands: a thread covers LA consecutive k of
+  // one row; row-fast operands: LA consecutive rows of one k.
   int a_r, a_k, b_r, b_k;
   if (AK) { a_r = tid / (BK / LA); a_k = (tid % (BK / LA)) * LA; }
   else { a_k = tid / (BM / LA); a_r = (tid % (BM / LA)) * LA; }
@@ -104,6 +119,13 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
 
   T ra[NP][LA], rb[NP][LB];
   auto load_a = [&](int64_t k0) {
+    const bool vec = !CPLX && AK && interior_mn && (k0 + BK <= p.k) && p.vec_a;
+    if (vec) {
+      const T *src = A + (m0 + a_r) * p.lda + k0 + a_k;
+#pragma unroll
+      for (int i = 0; i < LA; i += 16 / (int)sizeof(T)) *reinterpret_cast<float4 *>(&ra[0][i]) = *reinterpret_cast<const float4 *>(src + i);
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < LA; ++i) {
       const int64_t r = m0 + a_r + (AK ? 0 : i), kk = k0 + a_k + (AK ? i : 0);
@@ -119,6 +141,13 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
     }
   };
   auto load_b = [&](int64_t k0) {
+    const bool vec = !CPLX && BKF && interior_mn && (k0 + BK <= p.k) && p.vec_b;
+    if (vec) {
+      const T *src = B + (n0 + b_r) * p.ldb + k0 + b_k;
+#pragma unroll
+      for (int i = 0; i < LB; i += 16 / (int)sizeof(T)) *reinterpret_cast<float4 *>(&rb[0][i]) = *reinterpret_cast<const float4 *>(src + i);
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < LB; ++i) {
       const int64_t r = n0 + b_r + (BKF ? 0 : i), kk = k0 + b_k + (BKF ? i : 0);
@@ -136,15 +165,21 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
   auto store = [&](int buf) {
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
+      if (AK) {
 #pragma unroll
-      for (int i = 0; i < LA; ++i) {
-        if (AK) As[buf][q][a_k + i][a_r] = ra[q][i];
-        else As[buf][q][a_k][a_r + i] = ra[q][i];
+        for (int i = 0; i < LA; i += 16 / (int)sizeof(T))
+          *reinterpret_cast<float4 *>(&As[buf][q][a_r][a_k + i]) = *reinterpret_cast<const float4 *>(&ra[q][i]);
+      } else {
+#pragma unroll
+        for (int i = 0; i < LA; ++i) As[buf][q][a_r + i][a_k] = ra[q][i];
       }
+      if (BKF) {
 #pragma unroll
-      for (int i = 0; i < LB; ++i) {
-        if (BKF) Bs[buf][q][b_k + i][b_r] = rb[q][i];
-        else Bs[buf][q][b_k][b_r + i] = rb[q][i];
+        for (int i = 0; i < LB; i += 16 / (int)sizeof(T))
+          *reinterpret_cast<float4 *>(&Bs[buf][q][b_r][b_k + i]) = *reinterpret_cast<const float4 *>(&rb[q][i]);
+      } else {
+#pragma unroll
+        for (int i = 0; i < LB; ++i) Bs[buf][q][b_r + i][b_k] = rb[q][i];
       }
     }
   };
@@ -162,7 +197,7 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
   load_b(0);
   store(0);
   __syncthreads();
-  const int fr = lane & 15, fk = lane >> 4;
+  const int fr = lane & 15, fg = (lane >> 4) * 4;
   for (int64_t kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nk) {
@@ -170,27 +205,35 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
       load_b((kt + 1) * BK);
     }
 #pragma unroll
-    for (int ks = 0; ks < BK; ks += 4) {
-      T fa[NP][FM], fb[NP][FN];
+    for (int ks = 0; ks < BK; ks += 16) {
+      F4 fa[NP][FM], fb[NP][FN];
 #pragma unroll
       for (int q = 0; q < NP; ++q) {
 #pragma unroll
-        for (int i = 0; i < FM; ++i) fa[q][i] = As[cur][q][ks + fk][wm + i * 16 + fr];
+        for (int i = 0; i < FM; ++i) fa[q][i] = *reinterpret_cast<const F4 *>(&As[cur][q][wm + i * 16 + fr][ks + fg]);
 #pragma unroll
-        for (int j = 0; j < FN; ++j) fb[q][j] = Bs[cur][q][ks + fk][wn + j * 16 + fr];
+        for (int j = 0; j < FN; ++j) fb[q][j] = *reinterpret_cast<const F4 *>(&Bs[cur][q][wn + j * 16 + fr][ks + fg]);
       }
 #pragma unroll
-      for (int i = 0; i < FM; ++i) {
+      for (int st = 0; st < 4; ++st) {
 #pragma unroll
-        for (int j = 0; j < FN; ++j) {
-          if (CPLX) {
-            // (ar + i ai)(br + i bi) = (ar br - ai bi) + i (ar bi + ai br)
-            acc[0][i][j] = mfma(fa[0][i], fb[0][j], acc[0][i][j]);
-            acc[0][i][j] = mfma(-fa[1 % NP][i], fb[1 % NP][j], acc[0][i][j]);
-            acc[1 % NP][i][j] = mfma(fa[0][i], fb[1 % NP][j], acc[1 % NP][i][j]);
-            acc[1 % NP][i][j] = mfma(fa[1 % NP][i], fb[0][j], acc[1 % NP][i][j]);
-          } else {
-            acc[0][i][j] = mfma(fa[0][i], fb[0][j], acc[0][i][j]);
+        for (int i = 0; i < FM; ++i) {
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            const T a0 = st == 0 ? fa[0][i].x : st == 1 ? fa[0][i].y : st == 2 ? fa[0][i].z : fa[0][i].w;
+            const T b0 = st == 0 ? fb[0][j].x : st == 1 ? fb[0][j].y : st == 2 ? fb[0][j].z : fb[0][j].w;
+            if (CPLX) {
+              const F4 &ai4 = fa[1 % NP][i], &bi4 = fb[1 % NP][j];
+              const T a1 = st == 0 ? ai4.x : st == 1 ? ai4.y : st == 2 ? ai4.z : ai4.w;
+              const T b1 = st == 0 ? bi4.x : st == 1 ? bi4.y : st == 2 ? bi4.z : bi4.w;
+              // (ar + i ai)(br + i bi) = (ar br - ai bi) + i (ar bi + ai br)
+              acc[0][i][j] = mfma(a0, b0, acc[0][i][j]);
+              acc[0][i][j] = mfma(-a1, b1, acc[0][i][j]);
+              acc[1 % NP][i][j] = mfma(a0, b1, acc[1 % NP][i][j]);
+              acc[1 % NP][i][j] = mfma(a1, b0, acc[1 % NP][i][j]);
+            } else {
+              acc[0][i][j] = mfma(a0, b0, acc[0][i][j]);
+            }
           }
         }
       }
@@ -201,6 +244,7 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
   // epilogue: C = alpha * acc + beta * C (beta 0 or 1)
   T *C = static_cast<T *>(p.c);
   const T ar = (T)p.alpha_re, ai = (T)p.alpha_im;
+  const int fk = lane >> 4;
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
 #pragma unroll
@@ -504,6 +548,11 @@ int launch_gemm_ws(const GemmArgs &g, void *ws, hipStream_t s)
   p.beta_zero = g.beta_zero ? 1 : 0;
   p.conj_a = g.conj_a; p.conj_b = g.conj_b;
   p.xcd_remap = 1;
+  {
+    const int es = kind_size(g.kind);
+    p.vec_a = (((uintptr_t)g.a % 16) == 0) && ((p.lda * es) % 16 == 0);
+    p.vec_b = (((uintptr_t)g.b % 16) == 0) && ((p.ldb * es) % 16 == 0);
+  }
   if ((p.m + 63) / 64 * ((p.n + 63) / 64) >= (1ll << 31)) return TALSH_LIMIT_EXCEEDED;
   const int tile = g.tile < 0 ? 0 : g.tile;
   switch (g.kind) {

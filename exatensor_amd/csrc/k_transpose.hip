@@ -64,35 +64,78 @@ struct TileParams {
   uint32_t vol_tile;               // elements per tile
   uint32_t ntiles;
   DevDiv pad;                      // LDS slot = e + e / pad.d (pad.d = 0: no padding)
+  DevDiv tv_in0, tv_out0;          // first tile extent on each side divided by the vector width
 };
 
-template <typename T, bool CONJ>
-__device__ __forceinline__ T conj_if(T v) { return v; }
+// Elements move as integer words of their size (pure data movement, so the
+// result is bit-exact by construction); conjugation flips the sign bit of the
+// imaginary half of a complex word.
+template <typename W, bool CONJ>
+__device__ __forceinline__ W conj_if(W v) { return v; }
 template <>
-__device__ __forceinline__ float2 conj_if<float2, true>(float2 v) { v.y = -v.y; return v; }
+__device__ __forceinline__ uint2 conj_if<uint2, true>(uint2 v) { v.y ^= 0x80000000u; return v; }
 template <>
-__device__ __forceinline__ double2 conj_if<double2, true>(double2 v) { v.y = -v.y; return v; }
+__device__ __forceinline__ uint4 conj_if<uint4, true>(uint4 v) { v.w ^= 0x80000000u; return v; }
 
-template <typename T, int EPT, bool CONJ, typename OffT, bool RAG>
-__global__ __launch_bounds__(kNT) void k_permute_tiled(const T *__restrict__ in, T *__restrict__ out,
+template <typename W, int VEC>
+struct __align__(sizeof(W) * VEC) Pack { W w[VEC]; };
+
+// Streaming (nontemporal) global accesses: every element is read once and
+// written once, so the data should not displace useful lines from L2/MALL.
+typedef unsigned int nt1 __attribute__((ext_vector_type(1)));
+typedef unsigned int nt2 __attribute__((ext_vector_type(2)));
+typedef unsigned int nt4 __attribute__((ext_vector_type(4)));
+template <int BYTES>
+struct NtWord;
+template <>
+struct NtWord<4> { typedef unsigned int type; };
+template <>
+struct NtWord<8> { typedef nt2 type; };
+template <>
+struct NtWord<16> { typedef nt4 type; };
+
+template <typename P>
+__device__ __forceinline__ P ld_stream(const P *p)
+{
+  typedef typename NtWord<sizeof(P)>::type U;
+  const U u = __builtin_nontemporal_load(reinterpret_cast<const U *>(p));
+  return __builtin_bit_cast(P, u);
+}
+template <typename P>
+__device__ __forceinline__ void st_stream(P *p, const P &v)
+{
+  typedef typename NtWord<sizeof(P)>::type U;
+  __builtin_nontemporal_store(__builtin_bit_cast(U, v), reinterpret_cast<U *>(p));
+}
+
+// One workgroup walks tiles (persistent grid). Each thread owns EPT vectors of
+// VEC consecutive elements along the input-minor dimension for the load and
+// along the output-minor dimension for the store, so global accesses are
+// 16-byte wide whenever both minor extents allow it.
+template <typename W, int VEC, int EPT, bool CONJ, typename OffT, bool RAG>
+__global__ __launch_bounds__(kNT) void k_permute_tiled(const W *__restrict__ in, W *__restrict__ out,
                                                         const TileParams p)
 {
+  typedef Pack<W, VEC> P;
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  T *lds = reinterpret_cast<T *>(lds_raw);
+  W *lds = reinterpret_cast<W *>(lds_raw);
   const uint32_t tid = threadIdx.x;
+  const uint32_t nvec = p.vol_tile / VEC;
   OffT in_off[EPT], out_off[EPT];
-  uint32_t st_slot[EPT];
+  uint32_t st_ls[EPT];
   uint32_t rin[RAG ? EPT : 1], rout[RAG ? EPT : 1];
-  // one-time decode of this thread's tile elements (kept in registers)
+  // one-time decode of this thread's tile vectors (kept in registers)
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
-    const uint32_t e = tid + (uint32_t)k * kNT;
-    uint32_t r = e, cin[2] = {0, 0}, cout[2] = {0, 0};
+    const uint32_t v = tid + (uint32_t)k * kNT;
+    uint32_t r = v, cin[2] = {0, 0}, cout[2] = {0, 0};
     OffT io = 0, oo = 0;
     uint32_t ls = 0;
     for (int j = 0; j < p.nt; ++j) {
-      const uint32_t q = fdiv(r, p.t_in[j]);
-      const uint32_t c = r - q * p.t_in[j].d;
+      const DevDiv &dv = j == 0 ? p.tv_in0 : p.t_in[j];
+      const uint32_t q = fdiv(r, dv);
+      uint32_t c = r - q * dv.d;
+      if (j == 0) c *= VEC;
       r = q;
       io += (OffT)c * (OffT)p.is_in[j];
       if (RAG) {
@@ -100,10 +143,12 @@ __global__ __launch_bounds__(kNT) void k_permute_tiled(const T *__restrict__ in,
         if (p.nrag > 1 && p.rag_in[1] == j) cin[1] = c;
       }
     }
-    r = e;
+    r = v;
     for (int j = 0; j < p.nt; ++j) {
-      const uint32_t q = fdiv(r, p.t_out[j]);
-      const uint32_t c = r - q * p.t_out[j].d;
+      const DevDiv &dv = j == 0 ? p.tv_out0 : p.t_out[j];
+      const uint32_t q = fdiv(r, dv);
+      uint32_t c = r - q * dv.d;
+      if (j == 0) c *= VEC;
       r = q;
       oo += (OffT)c * (OffT)p.os_out[j];
       ls += c * p.ls_out[j];
@@ -114,17 +159,19 @@ __global__ __launch_bounds__(kNT) void k_permute_tiled(const T *__restrict__ in,
     }
     in_off[k] = io;
     out_off[k] = oo;
-    st_slot[k] = ls + fdiv(ls, p.pad);
+    st_ls[k] = ls;
     if (RAG) {
-      const bool valid = e < p.vol_tile;
+      const bool valid = v < nvec;
       rin[k] = valid ? (cin[0] | (cin[1] << 16)) : 0xFFFFFFFFu;
       rout[k] = valid ? (cout[0] | (cout[1] << 16)) : 0xFFFFFFFFu;
     }
   }
-  for (uint32_t tile = blockIdx.x; tile < p.ntiles; tile += gridDim.x) {
-    // decode the tile's base offsets (uniform across the workgroup)
-    OffT bin = 0, bout = 0;
-    uint32_t rem[2] = {0xFFFFu, 0xFFFFu};
+  const uint32_t step = p.ls_out[0];
+  // tile base decode (uniform across the workgroup)
+  auto decode = [&](uint32_t tile, OffT &bin, OffT &bout, uint32_t *rem, bool &full) {
+    bin = 0;
+    bout = 0;
+    rem[0] = rem[1] = 0xFFFFu;
     uint32_t r = tile;
     for (int g = 0; g < p.ng; ++g) {
       const uint32_t q = fdiv(r, p.gdiv[g]);
@@ -137,31 +184,69 @@ __global__ __launch_bounds__(kNT) void k_permute_tiled(const T *__restrict__ in,
         rem[p.g_rag[g]] = left < p.g_t[g] ? left : p.g_t[g];
       }
     }
-    const bool full = !RAG || ((rem[0] == p.rag_t[0]) && (p.nrag < 2 || rem[1] == p.rag_t[1]));
-    T v[EPT];
+    full = !RAG || ((rem[0] == p.rag_t[0]) && (p.nrag < 2 || rem[1] == p.rag_t[1]));
+  };
+  P val[EPT];
+  auto load = [&](OffT bin, const uint32_t *rem, bool full) {
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
-      const uint32_t e = tid + (uint32_t)k * kNT;
-      bool ok = e < p.vol_tile;
+      const uint32_t v = tid + (uint32_t)k * kNT;
+      bool ok = v < nvec;
       if (RAG && !full) ok = ok && (rin[k] & 0xFFFFu) < rem[0] && (rin[k] >> 16) < rem[1];
-      if (ok) v[k] = in[bin + in_off[k]];
+      if (ok) val[k] = ld_stream(reinterpret_cast<const P *>(in + (bin + in_off[k])));
     }
+  };
+  uint32_t tile = blockIdx.x;
+  if (tile >= p.ntiles) return;
+  OffT bin, bout;
+  uint32_t rem[2];
+  bool full;
+  decode(tile, bin, bout, rem, full);
+  load(bin, rem, full);
+  // software pipeline: the next tile's loads are in flight while the current
+  // tile drains from LDS to memory
+  while (true) {
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
-      const uint32_t e = tid + (uint32_t)k * kNT;
-      bool ok = e < p.vol_tile;
+      const uint32_t v = tid + (uint32_t)k * kNT;
+      bool ok = v < nvec;
       if (RAG && !full) ok = ok && (rin[k] & 0xFFFFu) < rem[0] && (rin[k] >> 16) < rem[1];
-      if (ok) lds[e + fdiv(e, p.pad)] = v[k];
+      if (ok) {
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) {
+          const uint32_t e = v * VEC + j;
+          lds[e + fdiv(e, p.pad)] = val[k].w[j];
+        }
+      }
     }
     __syncthreads();
+    const uint32_t next = tile + gridDim.x;
+    OffT nbin = 0, nbout = 0;
+    uint32_t nrem[2] = {0xFFFFu, 0xFFFFu};
+    bool nfull = true;
+    if (next < p.ntiles) {
+      decode(next, nbin, nbout, nrem, nfull);
+      load(nbin, nrem, nfull);
+    }
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const uint32_t f = tid + (uint32_t)k * kNT;
-      bool ok = f < p.vol_tile;
+      bool ok = f < nvec;
       if (RAG && !full) ok = ok && (rout[k] & 0xFFFFu) < rem[0] && (rout[k] >> 16) < rem[1];
-      if (ok) out[bout + out_off[k]] = conj_if<T, CONJ>(lds[st_slot[k]]);
+      if (ok) {
+        P o;
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) {
+          const uint32_t e = st_ls[k] + j * step;
+          o.w[j] = conj_if<W, CONJ>(lds[e + fdiv(e, p.pad)]);
+        }
+        st_stream(reinterpret_cast<P *>(out + (bout + out_off[k])), o);
+      }
     }
+    if (next >= p.ntiles) break;
     __syncthreads();
+    tile = next;
+    bin = nbin; bout = nbout; rem[0] = nrem[0]; rem[1] = nrem[1]; full = nfull;
   }
 }
 
@@ -194,6 +279,30 @@ __global__ __launch_bounds__(kNT) void k_copy(const T *__restrict__ in, T *__res
 {
   const uint64_t stride = (uint64_t)gridDim.x * kNT;
   for (uint64_t i = (uint64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += stride) out[i] = conj_if<T, CONJ>(in[i]);
+}
+
+// Identity permutation of a 16-byte aligned body: 16-byte streaming accesses,
+// four in flight per thread per iteration.
+template <typename W, bool CONJ>
+__global__ __launch_bounds__(kNT) void k_copy16(const nt4 *__restrict__ in, nt4 *__restrict__ out, uint64_t n16)
+{
+  constexpr int U = 4;
+  const uint64_t stride = (uint64_t)gridDim.x * kNT * U;
+  for (uint64_t base = (uint64_t)blockIdx.x * kNT * U + threadIdx.x; base < n16; base += stride) {
+    nt4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (base + (uint64_t)u * kNT < n16) v[u] = __builtin_nontemporal_load(in + base + (uint64_t)u * kNT);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (base + (uint64_t)u * kNT >= n16) continue;
+      if (CONJ) {  // complex words: flip the sign bit of each imaginary part
+        if (sizeof(W) == 8) { v[u].y ^= 0x80000000u; v[u].w ^= 0x80000000u; }
+        else v[u].w ^= 0x80000000u;
+      }
+      __builtin_nontemporal_store(v[u], out + base + (uint64_t)u * kNT);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -383,6 +492,7 @@ struct Planned {
   int ept = 0;
   size_t lds_bytes = 0;
   int grid = 0;
+  int vec = 1;        // elements per global access
   bool wide = false;  // 64-bit element offsets
   bool rag = false;   // masked edge tiles
   TilePlan tile;
@@ -522,8 +632,21 @@ int plan(int es, int rank, const int *dims, const int *o2n, Planned &pl, int cus
     if (cc < bestc) { bestc = cc; best = c; }
   }
   P.pad = to_dev(best);
+  // vector width: both minor dimensions (input dim 0, output dim 0) and their
+  // tile extents must be multiples of it so every vector is aligned and whole
+  int vec = 1;
+  for (int cand = 16 / es; cand >= 2; cand /= 2) {
+    const int i0 = tin[0], o0 = tout[0];
+    if (f.ext[i0] % cand == 0 && tp.t[i0] % cand == 0 && f.ext[o0] % cand == 0 && tp.t[o0] % cand == 0) {
+      vec = cand;
+      break;
+    }
+  }
+  pl.vec = vec;
+  P.tv_in0 = to_dev((uint32_t)(tp.t[tin[0]] / vec));
+  P.tv_out0 = to_dev((uint32_t)(tp.t[tout[0]] / vec));
   int ept = 1;
-  while ((long)ept * kNT < tp.vol) ept *= 2;
+  while ((long)ept * kNT * vec < tp.vol) ept *= 2;
   pl.ept = ept;
   pl.wide = vol >= (1ull << 32);
   pl.rag = P.nrag > 0;
@@ -534,16 +657,16 @@ int plan(int es, int rank, const int *dims, const int *o2n, Planned &pl, int cus
   return 0;
 }
 
-template <typename T, bool CONJ, typename OffT, bool RAG>
+template <typename W, int VEC, bool CONJ, typename OffT, bool RAG>
 int launch_tiled_v(const Planned &pl, const void *in, void *out, hipStream_t s)
 {
-  const T *a = static_cast<const T *>(in);
-  T *b = static_cast<T *>(out);
+  const W *a = static_cast<const W *>(in);
+  W *b = static_cast<W *>(out);
   dim3 g(pl.grid), blk(kNT);
   switch (pl.ept) {
-#define XT_CASE(E)                                                                                       \
-  case E:                                                                                                \
-    hipLaunchKernelGGL((k_permute_tiled<T, E, CONJ, OffT, RAG>), g, blk, pl.lds_bytes, s, a, b, pl.tp);  \
+#define XT_CASE(E)                                                                                             \
+  case E:                                                                                                      \
+    hipLaunchKernelGGL((k_permute_tiled<W, VEC, E, CONJ, OffT, RAG>), g, blk, pl.lds_bytes, s, a, b, pl.tp);   \
     break;
     XT_CASE(1) XT_CASE(2) XT_CASE(4) XT_CASE(8) XT_CASE(16)
 #undef XT_CASE
@@ -552,26 +675,47 @@ int launch_tiled_v(const Planned &pl, const void *in, void *out, hipStream_t s)
   return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
 }
 
-template <typename T, bool CONJ>
-int launch_tiled_t(const Planned &pl, const void *in, void *out, hipStream_t s)
+template <typename W, int VEC, bool CONJ>
+int launch_tiled_w(const Planned &pl, const void *in, void *out, hipStream_t s)
 {
-  if (pl.wide) return pl.rag ? launch_tiled_v<T, CONJ, uint64_t, true>(pl, in, out, s)
-                             : launch_tiled_v<T, CONJ, uint64_t, false>(pl, in, out, s);
-  return pl.rag ? launch_tiled_v<T, CONJ, uint32_t, true>(pl, in, out, s)
-                : launch_tiled_v<T, CONJ, uint32_t, false>(pl, in, out, s);
+  if (pl.wide) return pl.rag ? launch_tiled_v<W, VEC, CONJ, uint64_t, true>(pl, in, out, s)
+                             : launch_tiled_v<W, VEC, CONJ, uint64_t, false>(pl, in, out, s);
+  return pl.rag ? launch_tiled_v<W, VEC, CONJ, uint32_t, true>(pl, in, out, s)
+                : launch_tiled_v<W, VEC, CONJ, uint32_t, false>(pl, in, out, s);
 }
 
-template <typename T, bool CONJ>
+template <typename W, bool CONJ>
+int launch_tiled_t(const Planned &pl, const void *in, void *out, hipStream_t s)
+{
+  // only vector widths that keep one access <= 16 bytes are instantiated
+  if constexpr (sizeof(W) == 4) {
+    if (pl.vec == 4) return launch_tiled_w<W, 4, CONJ>(pl, in, out, s);
+    if (pl.vec == 2) return launch_tiled_w<W, 2, CONJ>(pl, in, out, s);
+  } else if constexpr (sizeof(W) == 8) {
+    if (pl.vec == 2) return launch_tiled_w<W, 2, CONJ>(pl, in, out, s);
+  }
+  return launch_tiled_w<W, 1, CONJ>(pl, in, out, s);
+}
+
+template <typename W, bool CONJ>
 int launch_copy_t(const void *in, void *out, uint64_t n, hipStream_t s, int cus)
 {
+  const uint64_t bytes = n * sizeof(W);
+  if (bytes % 16 == 0 && ((uintptr_t)in % 16) == 0 && ((uintptr_t)out % 16) == 0) {
+    const uint64_t n16 = bytes / 16;
+    const uint64_t blocks = std::min<uint64_t>((n16 + kNT * 4 - 1) / (kNT * 4), (uint64_t)cus * 8);
+    hipLaunchKernelGGL((k_copy16<W, CONJ>), dim3((unsigned)std::max<uint64_t>(blocks, 1)), dim3(kNT), 0, s,
+                       static_cast<const nt4 *>(in), static_cast<nt4 *>(out), n16);
+    return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
+  }
   const uint64_t blocks = std::min<uint64_t>((n + kNT - 1) / kNT, (uint64_t)cus * 16);
   if (blocks == 0) return 0;
-  hipLaunchKernelGGL((k_copy<T, CONJ>), dim3((unsigned)blocks), dim3(kNT), 0, s, static_cast<const T *>(in),
-                     static_cast<T *>(out), n);
+  hipLaunchKernelGGL((k_copy<W, CONJ>), dim3((unsigned)blocks), dim3(kNT), 0, s, static_cast<const W *>(in),
+                     static_cast<W *>(out), n);
   return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
 }
 
-template <typename T, bool CONJ>
+template <typename W, bool CONJ>
 int launch_scatter_t(const Fused &f, const void *in, void *out, hipStream_t s, int cus)
 {
   ScatterParams h;
@@ -589,17 +733,17 @@ int launch_scatter_t(const Fused &f, const void *in, void *out, hipStream_t s, i
     h.vol *= h.ext[i];
   }
   const uint64_t blocks = std::min<uint64_t>((h.vol + kNT - 1) / kNT, (uint64_t)cus * 16);
-  hipLaunchKernelGGL((k_permute_scatter<T, CONJ>), dim3((unsigned)std::max<uint64_t>(blocks, 1)), dim3(kNT), 0, s,
-                     static_cast<const T *>(in), static_cast<T *>(out), h);
+  hipLaunchKernelGGL((k_permute_scatter<W, CONJ>), dim3((unsigned)std::max<uint64_t>(blocks, 1)), dim3(kNT), 0, s,
+                     static_cast<const W *>(in), static_cast<W *>(out), h);
   return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
 }
 
-template <typename T, bool CONJ>
+template <typename W, bool CONJ>
 int dispatch(const Planned &pl, const void *in, void *out, uint64_t vol, hipStream_t s, int cus)
 {
-  if (pl.path == 0) return launch_copy_t<T, CONJ>(in, out, vol, s, cus);
-  if (pl.path == 1) return launch_tiled_t<T, CONJ>(pl, in, out, s);
-  return launch_scatter_t<T, CONJ>(pl.f, in, out, s, cus);
+  if (pl.path == 0) return launch_copy_t<W, CONJ>(in, out, vol, s, cus);
+  if (pl.path == 1) return launch_tiled_t<W, CONJ>(pl, in, out, s);
+  return launch_scatter_t<W, CONJ>(pl.f, in, out, s, cus);
 }
 
 }  // namespace
@@ -616,6 +760,7 @@ int plan_permute_info(int kind, int rank, const int *dims_in, const int *o2n, Pe
   info->in_run = pl.path == 1 ? pl.tile.in_run : 0;
   info->out_run = pl.path == 1 ? pl.tile.out_run : 0;
   info->grid = pl.grid;
+  info->vec = pl.path == 1 ? pl.vec : 1;
   return 0;
 }
 
@@ -637,16 +782,23 @@ int launch_permute(const void *in, void *out, int kind, int rank, const int *dim
   Planned pl;
   plan(es, rank, dims_in, o2n, pl, cus);
   if (pl.path == 1) {
-    // the persistent grid and per-thread element slots must cover the tile
-    if ((long)pl.ept * kNT < (long)pl.tp.vol_tile || pl.grid <= 0) return TALSH_FAILURE;
+    // vector accesses need both bodies aligned to the vector size
+    if (pl.vec > 1 && (((uintptr_t)in | (uintptr_t)out) % (uintptr_t)(pl.vec * es)) != 0) {
+      pl.tp.tv_in0 = to_dev(pl.tp.t_in[0].d);
+      pl.tp.tv_out0 = to_dev(pl.tp.t_out[0].d);
+      pl.vec = 1;
+      pl.ept = 1;
+      while ((long)pl.ept * kNT < (long)pl.tp.vol_tile) pl.ept *= 2;
+    }
+    // the persistent grid and per-thread vector slots must cover the tile
+    if ((long)pl.ept * kNT * pl.vec < (long)pl.tp.vol_tile || pl.grid <= 0 || pl.ept > kMaxEPT) return TALSH_FAILURE;
   }
   const bool cj = conj && kind_complex(kind);
   switch (kind) {
-  case R4: return dispatch<float, false>(pl, in, out, vol, s, cus);
-  case R8: return dispatch<double, false>(pl, in, out, vol, s, cus);
-  case C4: return cj ? dispatch<float2, true>(pl, in, out, vol, s, cus) : dispatch<float2, false>(pl, in, out, vol, s, cus);
-  case C8:
-    return cj ? dispatch<double2, true>(pl, in, out, vol, s, cus) : dispatch<double2, false>(pl, in, out, vol, s, cus);
+  case R4: return dispatch<uint32_t, false>(pl, in, out, vol, s, cus);
+  case R8: return dispatch<uint2, false>(pl, in, out, vol, s, cus);
+  case C4: return cj ? dispatch<uint2, true>(pl, in, out, vol, s, cus) : dispatch<uint2, false>(pl, in, out, vol, s, cus);
+  case C8: return cj ? dispatch<uint4, true>(pl, in, out, vol, s, cus) : dispatch<uint4, false>(pl, in, out, vol, s, cus);
   default: return TALSH_INVALID_ARGS;
   }
 }

@@ -21,6 +21,7 @@ struct PermuteInfo {
   int in_run, out_run;  // contiguous run lengths (elements) on each side
   int grid;             // workgroups launched
   int path;             // 0 copy, 1 tiled, 2 scatter fallback
+  int vec;              // elements per global access
 };
 int plan_permute_info(int kind, int rank, const int *dims_in, const int *o2n, PermuteInfo *info);
 

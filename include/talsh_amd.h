@@ -38,7 +38,8 @@ int talshAmdPlanCandidates(const char *cptrn, int drank, const int *ddims, int l
 /* Plan of the DLF permutation kernel for a tensor of data_kind with extents
  * dims[rank] and 0-based O2N permutation o2n[rank]: info[0] fused rank,
  * info[1] path (0 copy, 1 LDS-tiled, 2 scatter), info[2] tile volume,
- * info[3] input run, info[4] output run (elements), info[5] workgroups. */
+ * info[3] input run, info[4] output run (elements), info[5] workgroups,
+ * info[6] elements per global access. */
 int talshAmdPermutePlan(int data_kind, int rank, const int *dims, const int *o2n, int *info);
 
 /* Matrix product on GPU 0 with the engine's kernels (device pointers):
