@@ -194,6 +194,68 @@ def run_contractions(gb, reps=3):
     return out
 
 
+def run_blocksparse(gb, reps=3):
+    """C4: ~4096 uneven block contractions of D(a,b,c,d)+=L(a,k,c,l)*R(l,b,k,d)
+    (7 segments per mode, extents from {8..64}, 20% block fill), as one
+    persistent grouped launch replayed from a hipGraph, against one
+    talshTensorContract per block."""
+    from exatensor_amd import blocksparse as BS
+    T = gb.T
+    ext, ops = BS.generate()
+    out = {"ops": len(ops), "flops": BS.flops(ext, ops)}
+    for kind in (T.R8, T.R4):
+        tens = {}
+
+        def get(tag, key, modes, init):
+            if (tag, key) not in tens:
+                tens[(tag, key)] = gb.tensor(kind, BS.dims(ext, key, modes), init)
+            return tens[(tag, key)]
+        D = [get("D", o[0], "abcd", 0.0) for o in ops]
+        L = [get("L", o[1], "akcl", 0.01) for o in ops]
+        R = [get("R", o[2], "lbkd", 0.02) for o in ops]
+        gb.sync()
+        n = len(ops)
+        arr = lambda ts: (T._TP * n)(*[C.pointer(t.t) for t in ts])
+        b = C.c_void_p()
+        e = gb.L.talshAmdBatchCreate(BS.PATTERN.encode(), n, arr(D), arr(L), arr(R), 1.0, T.YEP, 0, C.byref(b))
+        if e:
+            raise RuntimeError(f"talshAmdBatchCreate: {e}")
+        gb.L.talshAmdBatchExecute(b, 1)
+        gb.L.talshAmdProfile(1)
+        gb.prof(T.KERNEL_GROUPED)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            gb.L.talshAmdBatchExecute(b, 1)
+        t1 = time.perf_counter()
+        kms, kn = gb.prof(T.KERNEL_GROUPED)
+        gb.L.talshAmdProfile(0)
+        gb.L.talshAmdBatchDestroy(b)
+        grouped_s = (t1 - t0) / reps
+        # baseline: one talshTensorContract (its own transposes + GEMM) per block
+        pat = BS.PATTERN.encode()
+        for i in range(min(64, n)):
+            gb.L.talshTensorContract(pat, D[i].ptr, L[i].ptr, R[i].ptr, 1.0, 0.0, gb.gpu, T.DEV_NULL, T.COPY_MTT,
+                                     T.YEP, None)
+        gb.sync()
+        t0 = time.perf_counter()
+        for i in range(n):
+            e = gb.L.talshTensorContract(pat, D[i].ptr, L[i].ptr, R[i].ptr, 1.0, 0.0, gb.gpu, T.DEV_NULL, T.COPY_MTT,
+                                         T.YEP, None)
+            if e:
+                raise RuntimeError(f"per-block contract {i}: {e}")
+        gb.sync()
+        per_block_s = time.perf_counter() - t0
+        name = "r8" if kind == T.R8 else "r4_fp32"
+        out[name] = dict(grouped_blocks_per_s=round(n / grouped_s, 1), grouped_ms=round(grouped_s * 1e3, 3),
+                         grouped_kernel_ms=round(kms / max(kn, 1), 3),
+                         grouped_tflops=round(out["flops"] / grouped_s / 1e12, 3),
+                         per_block_blocks_per_s=round(n / per_block_s, 1), per_block_ms=round(per_block_s * 1e3, 3),
+                         speedup=round(per_block_s / grouped_s, 2))
+        for t in tens.values():
+            t.destruct()
+    return out
+
+
 def cpu_baseline(target_seconds=12.0):
     """The reference TAL-SH CPU path (oracle/_ref, built from the reference's
     own sources) timed on this host's cores for a bounded sample of the same
@@ -264,6 +326,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-contract", action="store_true", help="skip the contraction components")
+    ap.add_argument("--no-blocksparse", action="store_true", help="skip the block-sparse component")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine runs one process's share of tensor blocks on one MI355X; "
@@ -272,6 +335,8 @@ def main():
     gb = GpuBench()
     tr = run_transposes(gb, args.steps, args.warmup)
     comps = {} if args.no_contract else run_contractions(gb)
+    if not args.no_blocksparse:
+        comps["blocksparse"] = run_blocksparse(gb)
     value = tr["bytes"] / 1e9 / tr["seconds"]
     achieved = tr["achieved_gbs"]
     line = {

@@ -129,6 +129,11 @@ def load_library(path=LIB_PATH):
                                    C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int,
                                    C.c_void_p, C.c_size_t]),
         "talshAmdGemmWorkspace": (C.c_size_t, [C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t]),
+        "talshAmdBatchCreate": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_TP), C.POINTER(_TP), C.POINTER(_TP),
+                                          C.c_double, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+        "talshAmdBatchExecute": (C.c_int, [C.c_void_p, C.c_int]),
+        "talshAmdBatchInfo": (C.c_int, [C.c_void_p, _IP, _IP, C.POINTER(C.c_double)]),
+        "talshAmdBatchDestroy": (C.c_int, [C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -53,6 +53,19 @@ int talshAmdGemm(int data_kind, int fast_math, size_t m, size_t n, size_t k, con
                  void *workspace, size_t workspace_bytes);
 size_t talshAmdGemmWorkspace(int data_kind, int fast_math, size_t m, size_t n, size_t k);
 
+/* Batched block-sparse contraction: nops block contractions
+ *   D_i (+)= alpha * L_i * R_i   (one symbolic pattern, R8 or R4, real alpha)
+ * whose bodies are all resident on GPU dev_num. Blocks sharing a D tensor
+ * accumulate in op order. Create builds the work queue once; Execute runs it
+ * synchronously as one persistent grouped launch (use_graph: replayed from a
+ * captured hipGraph); Destroy releases it. accumulative = NOPE overwrites
+ * each D block with the chain's sum. */
+int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens, talsh_tens_t *const *ltens,
+                        talsh_tens_t *const *rtens, double alpha_re, int accumulative, int dev_num, void **batch);
+int talshAmdBatchExecute(void *batch, int use_graph);
+int talshAmdBatchInfo(void *batch, int *nops, int *nitems, double *flops);
+int talshAmdBatchDestroy(void *batch);
+
 /* Synchronize all engine streams of a GPU (kind-specific number). */
 int talshAmdDeviceSynchronize(int dev_num);
 
