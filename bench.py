@@ -139,6 +139,37 @@ def run_transposes(gb, steps, warmup):
                 per_launch_bytes=per_launch_bytes, transposes_per_step=sum(len(w[3]) for w in work))
 
 
+def copy_probe(gb, reps=3):
+    """Identity copies of a 2 GiB R8 tensor (k_copy16: reads and writes
+    exactly 2 GiB each). Used only to calibrate hardware byte counters:
+    the HBM bytes a profiler's FETCH_SIZE/WRITE_SIZE report for this kernel
+    are known, so their ratio scales the counters of the transpose kernel."""
+    T = gb.T
+    src = gb.tensor(T.R8, [1 << 14, 1 << 14], 1.0)
+    dst = gb.tensor(T.R8, [1 << 14, 1 << 14], 0.0)
+    for _ in range(reps):
+        e = gb.L.talshTensorCopy(b"D(a,b)=L(a,b)", dst.ptr, src.ptr, gb.gpu, T.DEV_NULL, T.COPY_MT, None)
+        if e:
+            raise RuntimeError(f"identity copy -> {e}")
+    gb.sync()
+    src.destruct(); dst.destruct()
+
+
+def measured_traffic():
+    """HBM bytes per k_permute_tiled launch over the transpose workload, as
+    measured by hardware counters and recorded in
+    profiles/k_permute_tiled_transposes_traffic.md (None if absent)."""
+    p = os.path.join(ROOT, "profiles", "k_permute_tiled_transposes_traffic.md")
+    try:
+        with open(p) as f:
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch:"):
+                    return int(float(ln.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
 def run_contractions(gb, reps=3):
     T = gb.T
     out = {}
@@ -327,6 +358,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-contract", action="store_true", help="skip the contraction components")
     ap.add_argument("--no-blocksparse", action="store_true", help="skip the block-sparse component")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed region, run identity copies of known size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine runs one process's share of tensor blocks on one MI355X; "
@@ -334,6 +367,8 @@ def main():
         sys.exit(2)
     gb = GpuBench()
     tr = run_transposes(gb, args.steps, args.warmup)
+    if args.copy_probe:
+        copy_probe(gb)
     comps = {} if args.no_contract else run_contractions(gb)
     if not args.no_blocksparse:
         comps["blocksparse"] = run_blocksparse(gb)
@@ -357,7 +392,7 @@ def main():
                    "transposes_per_step": tr["transposes_per_step"], "bytes_per_step": tr["bytes"] // args.steps},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None, "kernel": "k_permute_tiled", "launches": tr["launches"],
+                     "traffic": measured_traffic(), "kernel": "k_permute_tiled", "launches": tr["launches"],
                      "avg_launch_ms": round(tr["kernel_ms"] / max(tr["launches"], 1), 4)},
         "components": comps,
     }
