@@ -225,6 +225,14 @@ def run_contractions(gb, reps=3):
     return out
 
 
+def _finish(gb, tk):
+    st = C.c_int()
+    e = gb.L.talshTaskWait(C.byref(tk), C.byref(st))
+    if e or st.value != gb.T.TALSH_TASK_COMPLETED:
+        raise RuntimeError(f"task wait: {e} status {st.value}")
+    gb.L.talshTaskDestruct(C.byref(tk))
+
+
 def run_blocksparse(gb, reps=3):
     """C4: ~4096 uneven block contractions of D(a,b,c,d)+=L(a,k,c,l)*R(l,b,k,d)
     (7 segments per mode, extents from {8..64}, 20% block fill), as one
@@ -262,18 +270,37 @@ def run_blocksparse(gb, reps=3):
         gb.L.talshAmdProfile(0)
         gb.L.talshAmdBatchDestroy(b)
         grouped_s = (t1 - t0) / reps
-        # baseline: one talshTensorContract (its own transposes + GEMM) per block
+        # baseline: one asynchronous talshTensorContract task (its own
+        # transposes + GEMM) per block, up to 64 tasks in flight over the
+        # engine's streams, as a block-level worker issues them
         pat = BS.PATTERN.encode()
-        for i in range(min(64, n)):
-            gb.L.talshTensorContract(pat, D[i].ptr, L[i].ptr, R[i].ptr, 1.0, 0.0, gb.gpu, T.DEV_NULL, T.COPY_MTT,
-                                     T.YEP, None)
+
+        def per_block(count):
+            window, inflight = 64, []
+            for i in range(count):
+                tk = T.TalshTask()
+                gb.L.talshTaskClean(C.byref(tk))
+                while True:
+                    e = gb.L.talshTensorContract(pat, D[i].ptr, L[i].ptr, R[i].ptr, 1.0, 0.0, gb.gpu, T.DEV_NULL,
+                                                 T.COPY_MTT, T.YEP, C.byref(tk))
+                    # an operand still held by an earlier task (blocks chained on one D): retire the oldest
+                    # task in flight and issue again
+                    if e in (T.TALSH_IN_PROGRESS, T.TRY_LATER) and inflight:
+                        _finish(gb, inflight.pop(0))
+                        continue
+                    break
+                if e:
+                    raise RuntimeError(f"per-block contract {i}: {e}")
+                inflight.append(tk)
+                if len(inflight) >= window:
+                    _finish(gb, inflight.pop(0))
+            for tk in inflight:
+                _finish(gb, tk)
+
+        per_block(min(64, n))
         gb.sync()
         t0 = time.perf_counter()
-        for i in range(n):
-            e = gb.L.talshTensorContract(pat, D[i].ptr, L[i].ptr, R[i].ptr, 1.0, 0.0, gb.gpu, T.DEV_NULL, T.COPY_MTT,
-                                         T.YEP, None)
-            if e:
-                raise RuntimeError(f"per-block contract {i}: {e}")
+        per_block(n)
         gb.sync()
         per_block_s = time.perf_counter() - t0
         name = "r8" if kind == T.R8 else "r4_fp32"

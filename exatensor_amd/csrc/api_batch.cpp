@@ -2,9 +2,10 @@
 //
 // A batch is built once from arrays of D/L/R block tensors that share one
 // contraction pattern (every tensor resident on the GPU) and can then be
-// executed many times, optionally as a captured hipGraph (counter reset +
-// persistent grouped kernel). Blocks that share a D tensor form a chain that
-// one work item accumulates in op order.
+// executed many times, optionally as a captured hipGraph (pack gather of the
+// distinct L and R blocks, counter reset, persistent grouped GEMM). Blocks
+// that share a D tensor form a chain that one work item accumulates in op
+// order.
 #include <algorithm>
 #include <map>
 #include <numeric>
@@ -27,18 +28,28 @@ GDiv make_div(uint32_t d)
 struct Batch {
   Device *dev = nullptr;
   int kind = NO_TYPE;
-  int nops = 0, nitems = 0;
+  int nops = 0, nitems = 0, njobs = 0;
+  int64_t nchunks = 0;
   double alpha = 1.0;
   bool acc = true;
-  int grid = 0;
+  int grid = 0, pack_grid = 0;
   double flops = 0;
-  talsh_dev_rsc_t mem{};     // ops + items + counter in the arena
+  talsh_dev_rsc_t mem{};     // ops + items + pack jobs + counter in the arena
+  talsh_dev_rsc_t packed{};  // packed L and R blocks
   BlockOp *d_ops = nullptr;
   WorkItem *d_items = nullptr;
+  PackJob *d_jobs = nullptr;
   int *d_counter = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
+
+int launch_batch(const Batch *B, hipStream_t s)
+{
+  int e = launch_pack(B->kind, B->d_jobs, B->njobs, B->nchunks, B->pack_grid, s);
+  if (e) return e;
+  return launch_grouped(B->kind, B->d_ops, B->d_items, B->nitems, B->d_counter, B->alpha, B->acc, B->grid, s);
+}
 
 void *gpu_body(talsh_tens_t *t, int flat, int kind)
 {
@@ -77,8 +88,31 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
       if (dig[i] == dig[lr + j]) return TALSH_INVALID_ARGS;  // hyper-index
   const int kind = nops > 0 && dtens[0] && dtens[0]->ndev > 0 ? dtens[0]->data_kind[0] : NO_TYPE;
   if (kind != R8 && kind != R4) return TALSH_NOT_IMPLEMENTED;
+  const int es = kind_size(kind);
   std::vector<BlockOp> ops(nops);
   std::vector<void *> dptr(nops);
+  std::vector<PackJob> jobs;
+  // (operand body, role 0 = L / 1 = R) -> element offset in the packed buffer
+  std::map<std::pair<const void *, int>, size_t> packed_at;
+  size_t packed_elems = 0;
+  auto add_job = [&](const void *src, int role, const IdxMap &rows, const IdxMap &cols, int64_t nrows, int64_t k) {
+    auto it = packed_at.find({src, role});
+    if (it != packed_at.end()) return it->second;
+    PackJob j{};
+    j.src = src;
+    j.rows = rows;
+    j.cols = cols;
+    j.nrows = nrows;
+    j.k = k;
+    j.kp = (k + 15) / 16 * 16;
+    j.dst = reinterpret_cast<void *>(packed_elems);  // offset for now, pointer after allocation
+    jobs.push_back(j);
+    packed_at[{src, role}] = packed_elems;
+    const size_t at = packed_elems;
+    packed_elems += ((size_t)(nrows * j.kp) + 31) / 32 * 32;  // 256-byte aligned sub-buffers (R8)
+    return at;
+  };
+  std::vector<size_t> a_at(nops), b_at(nops);
   double flops = 0;
   for (int o = 0; o < nops; ++o) {
     talsh_tens_t *D = dtens[o], *L = ltens[o], *R = rtens[o];
@@ -89,62 +123,71 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
     std::vector<int64_t> ds(dr), ls(lr), rs(rr);
     int64_t st = 1;
     for (int q = 0; q < dr; ++q) { ds[q] = st; st *= dd[q]; }
+    if (st >= (1ll << 31)) return TALSH_LIMIT_EXCEEDED;  // 32-bit offsets inside a D block
     st = 1;
     for (int q = 0; q < lr; ++q) { ls[q] = st; st *= ld[q]; }
     st = 1;
     for (int q = 0; q < rr; ++q) { rs[q] = st; st *= rd[q]; }
-    BlockOp &b = ops[o];
-    b.l = pl;
-    b.r = pr;
-    b.M.n = (int)mdims_l.size();
-    b.m = 1;
-    for (int q = 0; q < b.M.n; ++q) {
+    IdxMap Ml{}, Md{}, Nr{}, Nd{}, Kl{}, Kr{};
+    int64_t m = 1, n = 1, k = 1;
+    Ml.n = Md.n = (int)mdims_l.size();
+    for (int q = 0; q < Ml.n; ++q) {
       const int i = mdims_l[q];
       if (dd[dig[i] - 1] != ld[i]) return TALSH_INVALID_ARGS;
-      b.M.div[q] = make_div((uint32_t)ld[i]);
-      b.M.s0[q] = ls[i];
-      b.M.s1[q] = ds[dig[i] - 1];
-      b.m *= ld[i];
+      Ml.div[q] = Md.div[q] = make_div((uint32_t)ld[i]);
+      Ml.s[q] = ls[i];
+      Md.s[q] = ds[dig[i] - 1];
+      m *= ld[i];
     }
-    b.N.n = (int)ndims_r.size();
-    b.n = 1;
-    for (int q = 0; q < b.N.n; ++q) {
+    Nr.n = Nd.n = (int)ndims_r.size();
+    for (int q = 0; q < Nr.n; ++q) {
       const int j = ndims_r[q];
       if (dd[dig[lr + j] - 1] != rd[j]) return TALSH_INVALID_ARGS;
-      b.N.div[q] = make_div((uint32_t)rd[j]);
-      b.N.s0[q] = rs[j];
-      b.N.s1[q] = ds[dig[lr + j] - 1];
-      b.n *= rd[j];
+      Nr.div[q] = Nd.div[q] = make_div((uint32_t)rd[j]);
+      Nr.s[q] = rs[j];
+      Nd.s[q] = ds[dig[lr + j] - 1];
+      n *= rd[j];
     }
-    b.K.n = (int)kdims_l.size();
-    b.k = 1;
-    for (int q = 0; q < b.K.n; ++q) {
+    Kl.n = Kr.n = (int)kdims_l.size();  // contracted indices in L's order for both operands
+    for (int q = 0; q < Kl.n; ++q) {
       const int i = kdims_l[q], j = -dig[i] - 1;
       if (rd[j] != ld[i]) return TALSH_INVALID_ARGS;
-      b.K.div[q] = make_div((uint32_t)ld[i]);
-      b.K.s0[q] = ls[i];
-      b.K.s1[q] = rs[j];
-      b.k *= ld[i];
+      Kl.div[q] = Kr.div[q] = make_div((uint32_t)ld[i]);
+      Kl.s[q] = ls[i];
+      Kr.s[q] = rs[j];
+      k *= ld[i];
     }
-    if (b.m >= (1ll << 31) || b.n >= (1ll << 31) || b.k >= (1ll << 31)) return TALSH_LIMIT_EXCEEDED;
+    const int64_t kp = (k + 15) / 16 * 16;
+    if (m * kp >= (1ll << 31) || n * kp >= (1ll << 31)) return TALSH_LIMIT_EXCEEDED;
+    a_at[o] = add_job(pl, 0, Ml, Kl, m, k);
+    b_at[o] = add_job(pr, 1, Nr, Kr, n, k);
+    BlockOp &b = ops[o];
+    b.m = m;
+    b.n = n;
+    b.kp = kp;
+    b.Md = Md;
+    b.Nd = Nd;
     dptr[o] = pd;
-    flops += 2.0 * (double)b.m * (double)b.n * (double)b.k;
+    flops += 2.0 * (double)m * (double)n * (double)k;
+  }
+  int64_t nchunks = 0;
+  for (PackJob &j : jobs) {
+    j.chunk0 = nchunks;
+    nchunks += (j.nrows * j.kp + kPackChunk - 1) / kPackChunk;
   }
   // chains: ops grouped by D body, op order kept inside a chain
   std::vector<int> order(nops);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return dptr[a] < dptr[b]; });
-  std::vector<BlockOp> sorted(nops);
-  for (int i = 0; i < nops; ++i) sorted[i] = ops[order[i]];
   std::vector<WorkItem> items;
   std::vector<double> cost;
   for (int i = 0; i < nops;) {
     int j = i;
     while (j < nops && dptr[order[j]] == dptr[order[i]]) ++j;
-    const BlockOp &b = sorted[i];
+    const BlockOp &b = ops[order[i]];
     double kc = 0;
-    for (int q = i; q < j; ++q) kc += (double)sorted[q].k;
-    const int tm = (int)((b.m + 63) / 64), tn = (int)((b.n + 63) / 64);
+    for (int q = i; q < j; ++q) kc += (double)ops[order[q]].kp;
+    const int tm = (int)((b.m + kGroupedTile - 1) / kGroupedTile), tn = (int)((b.n + kGroupedTile - 1) / kGroupedTile);
     for (int a = 0; a < tm; ++a)
       for (int c = 0; c < tn; ++c) {
         items.push_back(WorkItem{i, j, a, c, dptr[order[i]]});
@@ -163,22 +206,43 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
   B->kind = kind;
   B->nops = nops;
   B->nitems = (int)sitems.size();
+  B->njobs = (int)jobs.size();
+  B->nchunks = nchunks;
   B->alpha = alpha_re;
   B->acc = accumulative != NOPE;
   B->flops = flops;
-  B->grid = std::min(B->nitems, d->num_cus * 4);  // 4 resident workgroups per CU (LDS-bound)
-  const size_t bytes = sizeof(BlockOp) * nops + sizeof(WorkItem) * sitems.size() + 256;
-  int e = image_alloc(d->flat_id, bytes, &B->mem);
+  B->grid = std::min(B->nitems, d->num_cus * 2);  // two resident workgroups per CU (LDS and registers)
+  B->pack_grid = (int)std::min<int64_t>(nchunks, (int64_t)d->num_cus * 8);
+  int e = image_alloc(d->flat_id, std::max<size_t>(packed_elems, 1) * es, &B->packed);
   if (e) { delete B; return e; }
+  char *pk = static_cast<char *>(B->packed.gmem_p);
+  std::vector<BlockOp> sorted(nops);
+  for (int i = 0; i < nops; ++i) {
+    sorted[i] = ops[order[i]];
+    sorted[i].a = pk + a_at[order[i]] * es;
+    sorted[i].b = pk + b_at[order[i]] * es;
+  }
+  for (PackJob &j : jobs) j.dst = pk + reinterpret_cast<size_t>(j.dst) * es;
+  const size_t ob = sizeof(BlockOp) * nops, ib = sizeof(WorkItem) * sitems.size(), jb = sizeof(PackJob) * jobs.size();
+  const size_t bytes = ob + ib + jb + 256;
+  e = image_alloc(d->flat_id, bytes, &B->mem);
+  if (e) {
+    image_release(B->packed, nullptr);
+    delete B;
+    return e;
+  }
   char *base = static_cast<char *>(B->mem.gmem_p);
   B->d_ops = reinterpret_cast<BlockOp *>(base);
-  B->d_items = reinterpret_cast<WorkItem *>(base + sizeof(BlockOp) * nops);
+  B->d_items = reinterpret_cast<WorkItem *>(base + ob);
+  B->d_jobs = reinterpret_cast<PackJob *>(base + ob + ib);
   B->d_counter = reinterpret_cast<int *>(base + bytes - 256);
   (void)hipSetDevice(d->ordinal);
-  bool ok = XT_HIP_OK(hipMemcpy(B->d_ops, sorted.data(), sizeof(BlockOp) * nops, hipMemcpyHostToDevice));
-  ok = ok && XT_HIP_OK(hipMemcpy(B->d_items, sitems.data(), sizeof(WorkItem) * sitems.size(), hipMemcpyHostToDevice));
+  bool ok = XT_HIP_OK(hipMemcpy(B->d_ops, sorted.data(), ob, hipMemcpyHostToDevice));
+  ok = ok && XT_HIP_OK(hipMemcpy(B->d_items, sitems.data(), ib, hipMemcpyHostToDevice));
+  ok = ok && XT_HIP_OK(hipMemcpy(B->d_jobs, jobs.data(), jb, hipMemcpyHostToDevice));
   if (!ok) {
     image_release(B->mem, nullptr);
+    image_release(B->packed, nullptr);
     delete B;
     return TALSH_FAILURE;
   }
@@ -199,7 +263,7 @@ int talshAmdBatchExecute(void *batch, int use_graph)
   if (use_graph) {
     if (!B->exec) {
       if (!XT_HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal))) return TALSH_FAILURE;
-      const int e = launch_grouped(B->kind, B->d_ops, B->d_items, B->nitems, B->d_counter, B->alpha, B->acc, B->grid, s);
+      const int e = launch_batch(B, s);
       hipGraph_t g = nullptr;
       const bool ok = XT_HIP_OK(hipStreamEndCapture(s, &g));
       if (e || !ok) return TALSH_FAILURE;
@@ -212,7 +276,7 @@ int talshAmdBatchExecute(void *batch, int use_graph)
     if (!ok) return TALSH_FAILURE;
   } else {
     prof_begin(s, kKGrouped, pr);
-    const int e = launch_grouped(B->kind, B->d_ops, B->d_items, B->nitems, B->d_counter, B->alpha, B->acc, B->grid, s);
+    const int e = launch_batch(B, s);
     prof_end(s, kKGrouped, pr);
     if (e) return e;
   }
@@ -240,6 +304,7 @@ int talshAmdBatchDestroy(void *batch)
   if (B->exec) (void)hipGraphExecDestroy(B->exec);
   if (B->graph) (void)hipGraphDestroy(B->graph);
   image_release(B->mem, nullptr);
+  image_release(B->packed, nullptr);
   delete B;
   return 0;
 }

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <vector>
 
+#include "contract_plan.hpp"
 #include "kernels.hpp"
 #include "runtime_impl.hpp"
 #include "talsh_amd.h"
@@ -34,7 +35,7 @@ int new_task(talsh_task_t *tt, Device *d, int kind, unsigned coh, int nargs)
   k->stream = d->streams[d->next_stream.fetch_add(1) % kStreamsPerDevice];
   bool ok = true;
   for (hipEvent_t *e : {&k->ev_start, &k->ev_comput, &k->ev_output, &k->ev_finish, &k->ev_mm0, &k->ev_mm1})
-    ok = ok && hipEventCreate(e) == hipSuccess;
+    ok = ok && (*e = event_get(true)) != nullptr;
   if (!ok) {
     delete k;
     return TALSH_FAILURE;
@@ -55,7 +56,7 @@ int new_task(talsh_task_t *tt, Device *d, int kind, unsigned coh, int nargs)
 void destroy_task_events(Task *k)
 {
   for (hipEvent_t e : {k->ev_start, k->ev_comput, k->ev_output, k->ev_finish, k->ev_mm0, k->ev_mm1})
-    if (e) (void)hipEventDestroy(e);
+    event_put(e, true);
 }
 
 // Prepared operand of an operation.
@@ -363,116 +364,6 @@ struct NoTemps {
 };
 
 }  // namespace
-
-// ---- transposition helper used by the planners --------------------------------
-// o2n (0-based) that reorders labels `from` into the order `to`.
-static std::vector<int> reorder(const std::vector<int> &from, const std::vector<int> &to)
-{
-  std::vector<int> o2n(from.size());
-  for (size_t i = 0; i < from.size(); ++i)
-    o2n[i] = (int)(std::find(to.begin(), to.end(), from[i]) - to.begin());
-  return o2n;
-}
-
-static bool is_identity(const std::vector<int> &p)
-{
-  for (size_t i = 0; i < p.size(); ++i)
-    if (p[i] != (int)i) return false;
-  return true;
-}
-
-static std::vector<int> concat(const std::vector<int> &a, const std::vector<int> &b)
-{
-  std::vector<int> c(a);
-  c.insert(c.end(), b.begin(), b.end());
-  return c;
-}
-
-// Contraction plan: how the operands are matricized for the GEMM.
-struct ContractPlan {
-  int kase = 0;                       // 0 scalar product, 1 scaled add, 2 full, 3 GEMM
-  bool swap = false;                  // GEMM computes D^T = R^T L^T
-  std::vector<int> l_o2n, r_o2n;      // transposes of L / R (empty: none)
-  std::vector<int> d_in_o2n, d_out_o2n;  // D -> Dmat and back (empty: none)
-  std::vector<int> lmat_dims, rmat_dims, dmat_dims;
-  bool a_kfast = true, b_kfast = true;
-  size_t m = 1, n = 1, k = 1;
-  double bytes_moved = 0;
-};
-
-// Heuristic plan: among the index orders that keep an operand in place
-// (contracted indices in L's or R's order, free indices in their own or D's
-// order, optionally computing D^T), pick the one moving the fewest bytes.
-static int plan_contraction(const std::vector<int> &dig, int dr, int lr, int rr, const std::vector<int> &dd,
-                            const std::vector<int> &ld, const std::vector<int> &rd, int es, bool acc,
-                            int variant, ContractPlan &P, std::vector<ContractPlan> *all)
-{
-  std::vector<int> llab(lr), rlab(rr), dlab(dr);
-  std::vector<int> ext(256, 0);
-  for (int q = 0; q < dr; ++q) { dlab[q] = q + 1; ext[q + 1] = dd[q]; }
-  for (int i = 0; i < lr; ++i) {
-    llab[i] = dig[i] > 0 ? dig[i] : 100 + i + 1;
-    if (dig[i] < 0) ext[100 + i + 1] = ld[i];
-  }
-  for (int j = 0; j < rr; ++j) rlab[j] = dig[lr + j] > 0 ? dig[lr + j] : 100 + (-dig[lr + j]);
-  std::vector<int> Kl, Kr, Ml, Nr, Md, Nd;
-  for (int i = 0; i < lr; ++i) (dig[i] < 0 ? Kl : Ml).push_back(llab[i]);
-  for (int j = 0; j < rr; ++j) (dig[lr + j] < 0 ? Kr : Nr).push_back(rlab[j]);
-  for (int lab : Ml)
-    if (std::find(Nr.begin(), Nr.end(), lab) != Nr.end()) return TALSH_INVALID_ARGS;  // hyper-index
-  for (int q = 0; q < dr; ++q) {
-    if (std::find(Ml.begin(), Ml.end(), dlab[q]) != Ml.end()) Md.push_back(dlab[q]);
-    else Nd.push_back(dlab[q]);
-  }
-  auto vol = [&](const std::vector<int> &labs) {
-    size_t v = 1;
-    for (int l : labs) v *= (size_t)ext[l];
-    return v;
-  };
-  for (int i = 0; i < lr; ++i) ext[llab[i]] = ld[i];
-  const size_t vl = vol(llab), vr = vol(rlab), vd = vol(dlab);
-  std::vector<ContractPlan> cands;
-  for (int ko = 0; ko < 2; ++ko)
-    for (int mo = 0; mo < 2; ++mo)
-      for (int no = 0; no < 2; ++no)
-        for (int sw = 0; sw < 2; ++sw) {
-          const std::vector<int> &K = ko ? Kr : Kl, &M = mo ? Md : Ml, &N = no ? Nd : Nr;
-          ContractPlan c;
-          c.kase = 3;
-          c.swap = sw;
-          c.m = vol(M); c.n = vol(N); c.k = vol(K);
-          double cost = 0;
-          const auto KM = concat(K, M), MK = concat(M, K), KN = concat(K, N), NK = concat(N, K);
-          bool lk = true, rk = true;
-          if (llab == KM) lk = true;
-          else if (llab == MK) lk = false;
-          else { c.l_o2n = reorder(llab, KM); cost += 2.0 * vl * es; }
-          if (rlab == KN) rk = true;
-          else if (rlab == NK) rk = false;
-          else { c.r_o2n = reorder(rlab, KN); cost += 2.0 * vr * es; }
-          const auto DT = sw ? concat(N, M) : concat(M, N);
-          if (dlab != DT) {
-            c.d_in_o2n = reorder(dlab, DT);
-            c.d_out_o2n = reorder(DT, dlab);
-            cost += (acc ? 4.0 : 2.0) * vd * es;
-            for (int lab : DT) c.dmat_dims.push_back(ext[lab]);
-          }
-          if (!sw) { c.a_kfast = lk; c.b_kfast = rk; }
-          else { c.a_kfast = rk; c.b_kfast = lk; }
-          c.bytes_moved = cost;
-          cands.push_back(c);
-        }
-  // deterministic preference on ties: fewer bytes, then k-fast operands, then no swap
-  auto score = [](const ContractPlan &c) {
-    return c.bytes_moved * 4.0 + (c.a_kfast ? 0 : 1) + (c.b_kfast ? 0 : 1) + (c.swap ? 0.5 : 0);
-  };
-  std::stable_sort(cands.begin(), cands.end(),
-                   [&](const ContractPlan &a, const ContractPlan &b) { return score(a) < score(b); });
-  if (all) *all = cands;
-  if (variant >= 0 && variant < (int)cands.size()) P = cands[variant];
-  else P = cands.front();
-  return 0;
-}
 
 }  // namespace xt
 
@@ -850,11 +741,7 @@ int talshTensorCopy(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens,
     int allocate(Temps &) { return 0; }
     int enqueue(Task *k, Device *)
     {
-      hipEvent_t pr[2];
-      prof_begin(k->stream, kKTranspose, pr);
-      const int e = launch_permute(L->exec, D->exec, L->kind, (int)ld.size(), ld.data(), o2n.data(), conj, k->stream);
-      prof_end(k->stream, kKTranspose, pr);
-      return e;
+      return launch_permute(L->exec, D->exec, L->kind, (int)ld.size(), ld.data(), o2n.data(), conj, k->stream);
     }
   } body{&args[0], &args[1], ld, o2n, lconj};
   const int kind = tensor_kind(dtens);
@@ -958,11 +845,19 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
   args[1].t = ltens;
   args[2].t = rtens;
   ContractPlan P;
+  int plan_variant = g_plan_variant, plan_tile = g_gemm_tile;
   if (dr == 0 && lr == 0 && rr == 0) P.kase = 0;
   else if ((lr == 0) != (rr == 0)) P.kase = 1;
   else if (dr == 0) P.kase = 2;
   else {
-    const int e = plan_contraction(dig, dr, lr, rr, dd, ld, rd, es, acc, g_plan_variant, P, nullptr);
+    if (plan_variant < 0) {  // learned selector, when one is installed
+      int v, t;
+      if (selector_choose(dig, dr, lr, rr, dd, ld, rd, kind, d->r4mode, acc, d->num_cus, &v, &t)) {
+        plan_variant = v;
+        if (plan_tile < 0) plan_tile = t;
+      }
+    }
+    const int e = plan_contraction(dig, dr, lr, rr, dd, ld, rd, es, acc, plan_variant, P, nullptr);
     if (e) return e;
   }
   size_t ll = 1, lrr = 1, lc = 1;
@@ -1047,24 +942,18 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
       const void *la = L->exec, *rb = R->exec;
       void *dc = D->exec;
       if (lt) {
-        prof_begin(s, kKTranspose, pr);
         int e = launch_permute(L->exec, lt, kind, lr, ld.data(), P.l_o2n.data(), false, s);
-        prof_end(s, kKTranspose, pr);
         if (e) return e;
         la = lt;
       }
       if (rt) {
-        prof_begin(s, kKTranspose, pr);
         int e = launch_permute(R->exec, rt, kind, rr, rd.data(), P.r_o2n.data(), false, s);
-        prof_end(s, kKTranspose, pr);
         if (e) return e;
         rb = rt;
       }
       if (dt) {
         if (acc) {
-          prof_begin(s, kKTranspose, pr);
           int e = launch_permute(D->exec, dt, kind, (int)dd.size(), dd.data(), P.d_in_o2n.data(), false, s);
-          prof_end(s, kKTranspose, pr);
           if (e) return e;
         }
         dc = dt;
@@ -1092,15 +981,13 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
       k->has_mm = true;
       if (e) return e;
       if (dt) {
-        prof_begin(s, kKTranspose, pr);
         e = launch_permute(dt, D->exec, kind, (int)P.dmat_dims.size(), P.dmat_dims.data(), P.d_out_o2n.data(), false,
                            s);
-        prof_end(s, kKTranspose, pr);
       }
       return e;
     }
   } body{&args[0], &args[1], &args[2], P, dig, dd, ld, rd, lr, rr, lconj, rconj, acc, sr, cplx ? si : 0.0,
-         g_gemm_tile};
+         plan_tile};
   if (!cplx) body.aim = si;  // real kinds: GEMM paths use Re(alpha); the scaled-add case needs the full factor
   if (!cplx && P.kase != 1) body.aim = 0.0;
   return run_task(task, d, (unsigned)copy_ctrl, args, kind, flops, body);

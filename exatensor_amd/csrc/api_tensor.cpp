@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <unordered_map>
 
@@ -13,6 +14,7 @@
 
 #include "kernels.hpp"
 #include "runtime_impl.hpp"
+#include "talsh_amd.h"
 
 namespace xt {
 
@@ -183,6 +185,46 @@ size_t tensor_volume(const talsh_tens_t *t) { return tensShape_volume(t->shape_p
 static std::mutex g_sync_mu;
 static std::unordered_map<void *, hipEvent_t> g_last_use;  // device image -> event of last task touching it
 
+// ---- event pool: hipEventCreate costs microseconds per call, and every task
+// uses several events; released events are kept per (device, timing) ------------
+static std::mutex g_evpool_mu;
+static std::map<std::pair<int, bool>, std::vector<hipEvent_t>> g_evpool;
+
+hipEvent_t event_get(bool timing)
+{
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  {
+    std::lock_guard<std::mutex> g(g_evpool_mu);
+    auto &v = g_evpool[{dev, timing}];
+    if (!v.empty()) {
+      hipEvent_t e = v.back();
+      v.pop_back();
+      return e;
+    }
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreateWithFlags(&e, timing ? hipEventDefault : hipEventDisableTiming) != hipSuccess) return nullptr;
+  return e;
+}
+
+void event_put(hipEvent_t e, bool timing)
+{
+  if (!e) return;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> g(g_evpool_mu);
+  g_evpool[{dev, timing}].push_back(e);
+}
+
+void event_pool_clear()
+{
+  std::lock_guard<std::mutex> g(g_evpool_mu);
+  for (auto &kv : g_evpool)
+    for (hipEvent_t e : kv.second) (void)hipEventDestroy(e);
+  g_evpool.clear();
+}
+
 void image_wait(hipStream_t s, void *img)
 {
   std::lock_guard<std::mutex> g(g_sync_mu);
@@ -192,13 +234,13 @@ void image_wait(hipStream_t s, void *img)
 
 void image_touch(hipStream_t s, void *img)
 {
-  hipEvent_t ev;
-  if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return;
+  hipEvent_t ev = event_get(false);
+  if (!ev) return;
   (void)hipEventRecord(ev, s);
   std::lock_guard<std::mutex> g(g_sync_mu);
   auto it = g_last_use.find(img);
   if (it != g_last_use.end()) {
-    (void)hipEventDestroy(it->second);
+    event_put(it->second, false);  // waits already enqueued on it keep their record
     it->second = ev;
   } else {
     g_last_use[img] = ev;
@@ -221,7 +263,14 @@ void prof_begin(hipStream_t s, int id, hipEvent_t *pair)
 {
   pair[0] = pair[1] = nullptr;
   if (!engine().profiling) return;
-  if (hipEventCreate(&pair[0]) != hipSuccess || hipEventCreate(&pair[1]) != hipSuccess) return;
+  pair[0] = event_get(true);
+  pair[1] = event_get(true);
+  if (!pair[0] || !pair[1]) {
+    event_put(pair[0], true);
+    event_put(pair[1], true);
+    pair[0] = pair[1] = nullptr;
+    return;
+  }
   (void)hipEventRecord(pair[0], s);
   (void)id;
 }
@@ -247,8 +296,8 @@ int prof_collect(int id, double *total_ms, long long *count)
       t += ms;
       ++c;
     }
-    (void)hipEventDestroy(r.a);
-    (void)hipEventDestroy(r.b);
+    event_put(r.a, true);
+    event_put(r.b, true);
   }
   g_prof.swap(keep);
   *total_ms = t;
@@ -317,6 +366,15 @@ int talshInit(size_t *host_buf_size, int *host_arg_max, int ngpus, int gpu_list[
   if (host_arg_max) *host_arg_max = 1 << 20;
   (void)host_buf_size;  // host images are allocated per tensor (pinned when a GPU is present)
   E.on = true;
+  if (const char *pm = std::getenv("TALSH_PLAN_MODEL")) {  // learned plan selector
+    void *h = nullptr;
+    if (pm[0] && talshAmdMlpLoad(pm, &h) == 0) {
+      talshAmdSelectorInstall(h);
+      talshAmdMlpDestroy(h);
+    } else if (pm[0]) {
+      log_error("TALSH_PLAN_MODEL=%s could not be loaded; contractions use the byte-count heuristic", pm);
+    }
+  }
   return TALSH_SUCCESS;
 }
 
@@ -331,6 +389,7 @@ int talshShutdown()
     if (hipDeviceSynchronize() != hipSuccess) err = TALSH_FAILURE;
   }
   image_forget_all();
+  event_pool_clear();
   for (Device *d : E.gpus) {
     for (auto &s : d->streams)
       if (s) (void)hipStreamDestroy(s);

@@ -7,30 +7,46 @@ namespace xt {
 
 struct GDiv { uint32_t d, m, s; };
 
-// A composite GEMM index over up to four tensor dimensions (fastest first),
-// with its strides in the two tensors that carry it.
+// A composite index over up to four tensor dimensions (fastest first) and its
+// strides in one tensor.
 struct IdxMap {
   int n;
   GDiv div[4];
-  int64_t s0[4], s1[4];
+  int64_t s[4];
 };
 
-// One block contraction: M maps (L, D), N maps (R, D), K maps (L, R).
+// Gather of one operand block into a packed k-fast matrix:
+//   dst[r * kp + kk] = src[rows(r) + cols(kk)]  (kk < k), 0 for k <= kk < kp.
+struct PackJob {
+  const void *src;
+  void *dst;
+  IdxMap rows, cols;
+  int64_t nrows, k, kp;
+  int64_t chunk0;  // index of this job's first chunk of kPackChunk elements
+};
+constexpr int kPackChunk = 4096;
+
+// One block product D(m x n) += A(m x k) B(n x k)^T on packed operands
+// (rows of kp elements, k fastest, zero padded); Md / Nd give the offsets of
+// the composite m / n indices in the D block.
 struct BlockOp {
-  const void *l;
-  const void *r;
-  IdxMap M, N, K;
-  int64_t m, n, k;
+  const void *a;
+  const void *b;
+  int64_t m, n, kp;
+  IdxMap Md, Nd;
 };
 
-// A 64x64 output tile of one D block and the chain ops[op_begin, op_end) that
-// accumulate into it, in order.
+// A tile of one D block and the chain ops[op_begin, op_end) that accumulate
+// into it, in order.
 struct WorkItem {
   int op_begin, op_end;
   int mt, nt;
   void *d;
 };
 
+constexpr int kGroupedTile = 128;
+
+int launch_pack(int kind, const PackJob *jobs, int njobs, int64_t nchunks, int grid, hipStream_t s);
 int launch_grouped(int kind, const BlockOp *ops, const WorkItem *items, int nitems, int *counter, double alpha,
                    bool accumulate, int grid, hipStream_t s);
 

@@ -534,6 +534,22 @@ size_t gemm_workspace_bytes(const GemmArgs &g)
 
 int launch_gemm(const GemmArgs &g, hipStream_t s) { return launch_gemm_ws(g, nullptr, s); }
 
+int gemm_tile_choices(int kind, R4Mode r4mode)
+{
+  if (kind == R8 || (kind == R4 && r4mode == R4Mode::kNativeF32)) return 3;
+  return 1;
+}
+
+void gemm_tile_shape(int kind, R4Mode r4mode, int tile, int *bm, int *bn, int *occupancy)
+{
+  *bm = *bn = 128;
+  *occupancy = 2;
+  if (kind == C8 || kind == C4) { *bm = *bn = 64; return; }
+  if (kind == R4 && r4mode == R4Mode::kBF16x3) { *occupancy = 1; return; }
+  if (tile == 1) *occupancy = 1;
+  if (tile == 2) *bm = *bn = 64;
+}
+
 int launch_gemm_ws(const GemmArgs &g, void *ws, hipStream_t s)
 {
   if (g.m == 0 || g.n == 0) return 0;

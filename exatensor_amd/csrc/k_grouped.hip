@@ -1,15 +1,21 @@
-// Persistent grouped contraction kernel for batches of uneven block
-// contractions (block-sparse tensors), gfx950.
+// Batched block-sparse contraction kernels for gfx950: a pack gather and a
+// persistent grouped GEMM over a device work queue.
 //
 // A batch holds thousands of block contractions D_i += alpha * L_i * R_i of
 // one pattern. Each is a GEMM whose row (m), column (n) and contracted (k)
-// indices are composites of up to four tensor dimensions; operands are read
-// in place from their DLF bodies through per-block index maps (fused gather:
-// no per-block transpose launches). Work items are 64x64 output tiles of a D
-// block together with the chain of (L, R) pairs that accumulate into it, so
-// every D element is produced by exactly one workgroup in a fixed order (no
-// atomics on D, deterministic results). Workgroups pull items from a device
-// work queue sorted by decreasing cost (longest first) until it is empty.
+// indices are composites of up to four tensor dimensions.
+//
+//  1. k_pack gathers every distinct L and R block once into a packed matrix
+//     with k fastest and k padded to a multiple of 16 with zeros, so that the
+//     GEMM reads 16-byte vectors along k with no bounds checks on k.
+//  2. k_grouped_gemm is persistent: each workgroup pulls work items (a
+//     128x128 tile of one D block plus the chain of (L, R) pairs that
+//     accumulate into it) from a device counter until the queue is empty.
+//     The inner loop is the dense engine GEMM's: double-buffered k-fast LDS
+//     tiles feeding 16x16x4 MFMAs (FP64 for R8, FP32 for R4). The epilogue
+//     scatters the tile into the D block through its index maps. Every D
+//     element is written by one workgroup in chain order: results are
+//     deterministic and need no atomics.
 //
 // Replaces the per-block talsh_tensor_contract calls issued by ExaTENSOR's
 // block-level caller (INTERVIRT/tavp_worker.F90:9440-9480).
@@ -42,35 +48,77 @@ __device__ __forceinline__ uint32_t gdiv(uint32_t n, const GDiv &f)
   return (f.d == 1) ? n : ((__umulhi(n, f.m) + n) >> f.s);
 }
 
-// Offsets of composite index v in the two tensors that carry it.
-__device__ __forceinline__ void map_offsets(const IdxMap &mp, uint32_t v, int64_t &o0, int64_t &o1)
+// Offset of composite index v in the tensor the map describes.
+__device__ __forceinline__ int64_t map_off(const IdxMap &mp, uint32_t v)
 {
-  o0 = 0;
-  o1 = 0;
+  int64_t o = 0;
   for (int q = 0; q < mp.n; ++q) {
     const uint32_t qq = gdiv(v, mp.div[q]);
-    const uint32_t c = v - qq * mp.div[q].d;
+    o += (int64_t)(v - qq * mp.div[q].d) * mp.s[q];
     v = qq;
-    o0 += (int64_t)c * mp.s0[q];
-    o1 += (int64_t)c * mp.s1[q];
   }
+  return o;
+}
+
+// 32-bit variant for D blocks (volume below 2^31, checked when the batch is built).
+__device__ __forceinline__ int32_t map_off32(const IdxMap &mp, uint32_t v)
+{
+  int32_t o = 0;
+  for (int q = 0; q < mp.n; ++q) {
+    const uint32_t qq = gdiv(v, mp.div[q]);
+    o += (int32_t)(v - qq * mp.div[q].d) * (int32_t)mp.s[q];
+    v = qq;
+  }
+  return o;
 }
 
 __global__ void k_zero_counter(int *c) { *c = 0; }
 
+// Packs chunks of kPackChunk destination elements; a chunk belongs to the job
+// with the largest chunk0 not above it (binary search over the sorted jobs).
 template <typename T>
-__global__ __launch_bounds__(256, 2) void k_grouped(const BlockOp *__restrict__ ops, const WorkItem *__restrict__ items,
-                                                     int nitems, int *__restrict__ counter, double alpha, int accumulate)
+__global__ __launch_bounds__(256) void k_pack(const PackJob *__restrict__ jobs, int njobs, int64_t nchunks)
 {
-  constexpr int BM = 64, BN = 64, BK = 16, KP = BK + (sizeof(T) == 4 ? 4 : 2);
+  for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].chunk0 <= c) lo = mid;
+      else hi = mid - 1;
+    }
+    const PackJob &J = jobs[lo];
+    const T *src = static_cast<const T *>(J.src);
+    T *dst = static_cast<T *>(J.dst);
+    const uint32_t total = (uint32_t)(J.nrows * J.kp), kp = (uint32_t)J.kp, k = (uint32_t)J.k;
+    const uint32_t base = (uint32_t)((c - J.chunk0) * kPackChunk);
+    for (int e = threadIdx.x; e < kPackChunk; e += 256) {
+      const uint32_t idx = base + (uint32_t)e;
+      if (idx >= total) break;
+      const uint32_t r = idx / kp, kk = idx - r * kp;
+      T v = T(0);
+      if (kk < k) v = src[map_off(J.rows, r) + map_off(J.cols, kk)];
+      dst[idx] = v;
+    }
+  }
+}
+
+template <typename T, int OCC>
+__global__ __launch_bounds__(256, OCC) void k_grouped_gemm(const BlockOp *__restrict__ ops,
+                                                           const WorkItem *__restrict__ items, int nitems,
+                                                           int *__restrict__ counter, double alpha, int accumulate)
+{
+  constexpr int BM = kGroupedTile, BN = kGroupedTile, BK = 16, KP = BK + (sizeof(T) == 4 ? 4 : 2);
+  constexpr int FM = BM / 32, FN = BN / 32;
+  constexpr int LA = BM * BK / 256;           // elements staged per thread and operand
+  constexpr int VW = 16 / (int)sizeof(T);     // elements per 16-byte access
   typedef typename GAcc<T>::type AccT;
   typedef typename GF4<T>::type F4;
   __shared__ __align__(16) T As[2][BM][KP];
   __shared__ __align__(16) T Bs[2][BN][KP];
   __shared__ int next_item;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-  const int sr = tid & 63, sk = (tid >> 6) * 4;  // staging: row sr, k = sk..sk+3
+  const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * (BN / 2);
+  const int s_r = tid / (BK / LA), s_k = (tid % (BK / LA)) * LA;  // staging row, first k
   const int fr = lane & 15, fg = (lane >> 4) * 4, fk = lane >> 4;
   while (true) {
     if (tid == 0) next_item = atomicAdd(counter, 1);
@@ -80,85 +128,84 @@ __global__ __launch_bounds__(256, 2) void k_grouped(const BlockOp *__restrict__ 
     if (it >= nitems) break;
     const WorkItem w = items[it];
     const int64_t m0 = (int64_t)w.mt * BM, n0 = (int64_t)w.nt * BN;
-    AccT acc[2][2];
+    AccT acc[FM][FN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = AccT{0, 0, 0, 0};
+      for (int j = 0; j < FN; ++j) acc[i][j] = AccT{0, 0, 0, 0};
     for (int oi = w.op_begin; oi < w.op_end; ++oi) {
       const BlockOp &op = ops[oi];
-      const T *L = static_cast<const T *>(op.l);
-      const T *R = static_cast<const T *>(op.r);
-      // this thread's staging row in A (m) and B (n): offsets into L and R
-      const int64_t am = m0 + sr, bn = n0 + sr;
-      int64_t offLm = 0, offRn = 0, dummy;
-      const bool mok = am < op.m, nok = bn < op.n;
-      if (mok) map_offsets(op.M, (uint32_t)am, offLm, dummy);
-      if (nok) map_offsets(op.N, (uint32_t)bn, offRn, dummy);
-      const int64_t nk = (op.k + BK - 1) / BK;
-      T ra[4], rb[4];
+      const int64_t kp = op.kp;
+      const bool aok = m0 + s_r < op.m, bok = n0 + s_r < op.n;
+      const T *ap = static_cast<const T *>(op.a) + (aok ? (m0 + s_r) * kp + s_k : 0);
+      const T *bp = static_cast<const T *>(op.b) + (bok ? (n0 + s_r) * kp + s_k : 0);
+      T ra[LA], rb[LA];
       auto load = [&](int64_t k0) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int64_t kk = k0 + sk + j;
-          int64_t offLk = 0, offRk = 0;
-          const bool kok = kk < op.k;
-          if (kok) map_offsets(op.K, (uint32_t)kk, offLk, offRk);
-          ra[j] = (mok && kok) ? L[offLm + offLk] : T(0);
-          rb[j] = (nok && kok) ? R[offRn + offRk] : T(0);
+        for (int i = 0; i < LA; i += VW) {
+          *reinterpret_cast<float4 *>(&ra[i]) =
+              aok ? *reinterpret_cast<const float4 *>(ap + k0 + i) : float4{0.f, 0.f, 0.f, 0.f};
+          *reinterpret_cast<float4 *>(&rb[i]) =
+              bok ? *reinterpret_cast<const float4 *>(bp + k0 + i) : float4{0.f, 0.f, 0.f, 0.f};
         }
       };
       auto store = [&](int buf) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          As[buf][sr][sk + j] = ra[j];
-          Bs[buf][sr][sk + j] = rb[j];
+        for (int i = 0; i < LA; i += VW) {
+          *reinterpret_cast<float4 *>(&As[buf][s_r][s_k + i]) = *reinterpret_cast<const float4 *>(&ra[i]);
+          *reinterpret_cast<float4 *>(&Bs[buf][s_r][s_k + i]) = *reinterpret_cast<const float4 *>(&rb[i]);
         }
       };
+      const int64_t nk = kp / BK;
       load(0);
       store(0);
       __syncthreads();
       for (int64_t kt = 0; kt < nk; ++kt) {
         const int cur = kt & 1;
         if (kt + 1 < nk) load((kt + 1) * BK);
-        F4 fa[2], fb[2];
+        F4 fa[FM], fb[FN];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) fa[i] = *reinterpret_cast<const F4 *>(&As[cur][wm + i * 16 + fr][fg]);
+        for (int i = 0; i < FM; ++i) fa[i] = *reinterpret_cast<const F4 *>(&As[cur][wm + i * 16 + fr][fg]);
 #pragma unroll
-        for (int j = 0; j < 2; ++j) fb[j] = *reinterpret_cast<const F4 *>(&Bs[cur][wn + j * 16 + fr][fg]);
+        for (int j = 0; j < FN; ++j) fb[j] = *reinterpret_cast<const F4 *>(&Bs[cur][wn + j * 16 + fr][fg]);
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int st = 0; st < 4; ++st)
 #pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            acc[i][j] = gmfma(fa[i].x, fb[j].x, acc[i][j]);
-            acc[i][j] = gmfma(fa[i].y, fb[j].y, acc[i][j]);
-            acc[i][j] = gmfma(fa[i].z, fb[j].z, acc[i][j]);
-            acc[i][j] = gmfma(fa[i].w, fb[j].w, acc[i][j]);
-          }
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+              const T a0 = st == 0 ? fa[i].x : st == 1 ? fa[i].y : st == 2 ? fa[i].z : fa[i].w;
+              const T b0 = st == 0 ? fb[j].x : st == 1 ? fb[j].y : st == 2 ? fb[j].z : fb[j].w;
+              acc[i][j] = gmfma(a0, b0, acc[i][j]);
+            }
         if (kt + 1 < nk) store(cur ^ 1);
         __syncthreads();
       }
     }
-    // write the tile of D (strides of the chain's D block from its first op)
-    const BlockOp &op0 = ops[w.op_begin];
+    // scatter the tile into the D block (index maps of the chain's first op)
+    const BlockOp &o0 = ops[w.op_begin];
     T *D = static_cast<T *>(w.d);
+    int32_t coff[FN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < FN; ++j) {
+      const int64_t col = n0 + wn + j * 16 + fr;
+      coff[j] = col < o0.n ? map_off32(o0.Nd, (uint32_t)col) : -1;
+    }
+    const T a = (T)alpha;
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int64_t col = n0 + wn + j * 16 + fr;
-        if (col >= op0.n) continue;
-        int64_t dn, d0;
-        map_offsets(op0.N, (uint32_t)col, d0, dn);
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int64_t row = m0 + wm + i * 16 + (sizeof(T) == 8 ? fk + 4 * r : fk * 4 + r);
-          if (row >= op0.m) continue;
-          int64_t dm, l0;
-          map_offsets(op0.M, (uint32_t)row, l0, dm);
-          T v = (T)alpha * acc[i][j][r];
-          if (accumulate) v += D[dm + dn];
-          D[dm + dn] = v;
+      for (int r = 0; r < 4; ++r) {
+        // FP64 16x16x4 keeps rows lane/16 + 4r in a lane, FP32 rows 4(lane/16) + r
+        const int64_t row = m0 + wm + i * 16 + (sizeof(T) == 8 ? fk + 4 * r : fk * 4 + r);
+        if (row >= o0.m) continue;
+        const int32_t roff = map_off32(o0.Md, (uint32_t)row);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          if (coff[j] < 0) continue;
+          T v = a * acc[i][j][r];
+          if (accumulate) v += D[roff + coff[j]];
+          D[roff + coff[j]] = v;
         }
       }
   }
@@ -166,16 +213,25 @@ __global__ __launch_bounds__(256, 2) void k_grouped(const BlockOp *__restrict__ 
 
 }  // namespace
 
+int launch_pack(int kind, const PackJob *jobs, int njobs, int64_t nchunks, int grid, hipStream_t s)
+{
+  if (njobs <= 0 || nchunks <= 0) return 0;
+  if (kind == R8) hipLaunchKernelGGL(k_pack<double>, dim3(grid), dim3(256), 0, s, jobs, njobs, nchunks);
+  else if (kind == R4) hipLaunchKernelGGL(k_pack<float>, dim3(grid), dim3(256), 0, s, jobs, njobs, nchunks);
+  else return TALSH_INVALID_ARGS;
+  return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
+}
+
 int launch_grouped(int kind, const BlockOp *ops, const WorkItem *items, int nitems, int *counter, double alpha,
                    bool accumulate, int grid, hipStream_t s)
 {
   if (nitems <= 0) return 0;
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, counter);
   if (kind == R8)
-    hipLaunchKernelGGL(k_grouped<double>, dim3(grid), dim3(256), 0, s, ops, items, nitems, counter, alpha,
+    hipLaunchKernelGGL((k_grouped_gemm<double, 2>), dim3(grid), dim3(256), 0, s, ops, items, nitems, counter, alpha,
                        accumulate ? 1 : 0);
   else if (kind == R4)
-    hipLaunchKernelGGL(k_grouped<float>, dim3(grid), dim3(256), 0, s, ops, items, nitems, counter, alpha,
+    hipLaunchKernelGGL((k_grouped_gemm<float, 2>), dim3(grid), dim3(256), 0, s, ops, items, nitems, counter, alpha,
                        accumulate ? 1 : 0);
   else
     return TALSH_INVALID_ARGS;

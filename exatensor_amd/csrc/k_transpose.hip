@@ -18,6 +18,9 @@
 //    registers, so the per-tile work is one base-offset decode per workgroup
 //    plus one coalesced load and one coalesced store per element.
 #include <algorithm>
+#include <mutex>
+#include <string>
+#include <unordered_map>
 #include <cstring>
 #include <numeric>
 
@@ -780,7 +783,26 @@ int launch_permute(const void *in, void *out, int kind, int rank, const int *dim
   if (in == out) return TALSH_INVALID_ARGS;  // out-of-place only
   static int cus = num_cus_current();
   Planned pl;
-  plan(es, rank, dims_in, o2n, pl, cus);
+  {
+    // plans are memoized per (element size, extents, permutation): planning
+    // (fusion, tile choice, bank-conflict simulation) costs far more host time
+    // than a launch
+    static std::mutex mu;
+    static std::unordered_map<std::string, Planned> cache;
+    std::string key;
+    key.reserve(8 * rank + 8);
+    key.append(std::to_string(es)).push_back(':');
+    for (int i = 0; i < rank; ++i) key.append(std::to_string(dims_in[i])).append(",").append(std::to_string(o2n[i])).push_back(';');
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      pl = it->second;
+    } else {
+      plan(es, rank, dims_in, o2n, pl, cus);
+      if (cache.size() >= 4096) cache.clear();
+      cache.emplace(key, pl);
+    }
+  }
   if (pl.path == 1) {
     // vector accesses need both bodies aligned to the vector size
     if (pl.vec > 1 && (((uintptr_t)in | (uintptr_t)out) % (uintptr_t)(pl.vec * es)) != 0) {
@@ -794,13 +816,18 @@ int launch_permute(const void *in, void *out, int kind, int rank, const int *dim
     if ((long)pl.ept * kNT * pl.vec < (long)pl.tp.vol_tile || pl.grid <= 0 || pl.ept > kMaxEPT) return TALSH_FAILURE;
   }
   const bool cj = conj && kind_complex(kind);
+  hipEvent_t pr[2];
+  prof_begin(s, kKTranspose, pr);  // timing brackets the launch only
+  int e = TALSH_INVALID_ARGS;
   switch (kind) {
-  case R4: return dispatch<uint32_t, false>(pl, in, out, vol, s, cus);
-  case R8: return dispatch<uint2, false>(pl, in, out, vol, s, cus);
-  case C4: return cj ? dispatch<uint2, true>(pl, in, out, vol, s, cus) : dispatch<uint2, false>(pl, in, out, vol, s, cus);
-  case C8: return cj ? dispatch<uint4, true>(pl, in, out, vol, s, cus) : dispatch<uint4, false>(pl, in, out, vol, s, cus);
-  default: return TALSH_INVALID_ARGS;
+  case R4: e = dispatch<uint32_t, false>(pl, in, out, vol, s, cus); break;
+  case R8: e = dispatch<uint2, false>(pl, in, out, vol, s, cus); break;
+  case C4: e = cj ? dispatch<uint2, true>(pl, in, out, vol, s, cus) : dispatch<uint2, false>(pl, in, out, vol, s, cus); break;
+  case C8: e = cj ? dispatch<uint4, true>(pl, in, out, vol, s, cus) : dispatch<uint4, false>(pl, in, out, vol, s, cus); break;
+  default: break;
   }
+  prof_end(s, kKTranspose, pr);
+  return e;
 }
 
 }  // namespace xt

@@ -55,5 +55,9 @@ int launch_gemm(const GemmArgs &g, hipStream_t s);
 // Bytes of scratch needed by launch_gemm (BF16x3 split planes); 0 otherwise.
 size_t gemm_workspace_bytes(const GemmArgs &g);
 int launch_gemm_ws(const GemmArgs &g, void *ws, hipStream_t s);
+// Number of tile-shape choices of launch_gemm for a kind, and the output tile
+// (bm x bn) and occupancy target of a choice.
+int gemm_tile_choices(int kind, R4Mode r4mode);
+void gemm_tile_shape(int kind, R4Mode r4mode, int tile, int *bm, int *bn, int *occupancy);
 
 }  // namespace xt

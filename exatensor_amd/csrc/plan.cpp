@@ -1,0 +1,320 @@
+// Contraction planning (TTGT matricization candidates), their features, and
+// the learned plan selector that picks among them.
+//
+// The reference fixes one matricization, D(ll x lr) += L^T(lc x ll) R(lc x lr)
+// with every operand transposed into that order
+// (TALSH/tensor_algebra_cpu.F90:5613-5755, get_contr_permutations), and fixed
+// heuristics elsewhere (TALSH/tensor_algebra_gpu_nvidia.cu:1188-1272,
+// TALSH/talshc.cpp:366-488). Here the GEMM reads either operand k-fast or
+// m/n-fast and may compute D^T, so several candidates exist; the byte-count
+// heuristic ranks them and an optional learned model (a small MLP predicting
+// the log of kernel time from the features below) chooses among candidate and
+// GEMM tile shape.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "contract_plan.hpp"
+#include "kernels.hpp"
+#include "mlp.hpp"
+#include "selector.hpp"
+
+namespace xt {
+
+// o2n (0-based) that reorders labels `from` into the order `to`.
+std::vector<int> reorder(const std::vector<int> &from, const std::vector<int> &to)
+{
+  std::vector<int> o2n(from.size());
+  for (size_t i = 0; i < from.size(); ++i)
+    o2n[i] = (int)(std::find(to.begin(), to.end(), from[i]) - to.begin());
+  return o2n;
+}
+
+bool is_identity(const std::vector<int> &p)
+{
+  for (size_t i = 0; i < p.size(); ++i)
+    if (p[i] != (int)i) return false;
+  return true;
+}
+
+static std::vector<int> concat(const std::vector<int> &a, const std::vector<int> &b)
+{
+  std::vector<int> c(a);
+  c.insert(c.end(), b.begin(), b.end());
+  return c;
+}
+
+// Heuristic plan: among the index orders that keep an operand in place
+// (contracted indices in L's or R's order, free indices in their own or D's
+// order, optionally computing D^T), pick the one moving the fewest bytes.
+int plan_contraction(const std::vector<int> &dig, int dr, int lr, int rr, const std::vector<int> &dd,
+                     const std::vector<int> &ld, const std::vector<int> &rd, int es, bool acc, int variant,
+                     ContractPlan &P, std::vector<ContractPlan> *all)
+{
+  std::vector<int> llab(lr), rlab(rr), dlab(dr);
+  std::vector<int> ext(256, 0);
+  for (int q = 0; q < dr; ++q) { dlab[q] = q + 1; ext[q + 1] = dd[q]; }
+  for (int i = 0; i < lr; ++i) {
+    llab[i] = dig[i] > 0 ? dig[i] : 100 + i + 1;
+    if (dig[i] < 0) ext[100 + i + 1] = ld[i];
+  }
+  for (int j = 0; j < rr; ++j) rlab[j] = dig[lr + j] > 0 ? dig[lr + j] : 100 + (-dig[lr + j]);
+  std::vector<int> Kl, Kr, Ml, Nr, Md, Nd;
+  for (int i = 0; i < lr; ++i) (dig[i] < 0 ? Kl : Ml).push_back(llab[i]);
+  for (int j = 0; j < rr; ++j) (dig[lr + j] < 0 ? Kr : Nr).push_back(rlab[j]);
+  for (int lab : Ml)
+    if (std::find(Nr.begin(), Nr.end(), lab) != Nr.end()) return TALSH_INVALID_ARGS;  // hyper-index
+  for (int q = 0; q < dr; ++q) {
+    if (std::find(Ml.begin(), Ml.end(), dlab[q]) != Ml.end()) Md.push_back(dlab[q]);
+    else Nd.push_back(dlab[q]);
+  }
+  auto vol = [&](const std::vector<int> &labs) {
+    size_t v = 1;
+    for (int l : labs) v *= (size_t)ext[l];
+    return v;
+  };
+  for (int i = 0; i < lr; ++i) ext[llab[i]] = ld[i];
+  const size_t vl = vol(llab), vr = vol(rlab), vd = vol(dlab);
+  std::vector<ContractPlan> cands;
+  for (int ko = 0; ko < 2; ++ko)
+    for (int mo = 0; mo < 2; ++mo)
+      for (int no = 0; no < 2; ++no)
+        for (int sw = 0; sw < 2; ++sw) {
+          const std::vector<int> &K = ko ? Kr : Kl, &M = mo ? Md : Ml, &N = no ? Nd : Nr;
+          ContractPlan c;
+          c.kase = 3;
+          c.swap = sw;
+          c.m = vol(M); c.n = vol(N); c.k = vol(K);
+          double cost = 0;
+          const auto KM = concat(K, M), MK = concat(M, K), KN = concat(K, N), NK = concat(N, K);
+          bool lk = true, rk = true;
+          if (llab == KM) lk = true;
+          else if (llab == MK) lk = false;
+          else { c.l_o2n = reorder(llab, KM); cost += 2.0 * vl * es; }
+          if (rlab == KN) rk = true;
+          else if (rlab == NK) rk = false;
+          else { c.r_o2n = reorder(rlab, KN); cost += 2.0 * vr * es; }
+          const auto DT = sw ? concat(N, M) : concat(M, N);
+          if (dlab != DT) {
+            c.d_in_o2n = reorder(dlab, DT);
+            c.d_out_o2n = reorder(DT, dlab);
+            cost += (acc ? 4.0 : 2.0) * vd * es;
+            for (int lab : DT) c.dmat_dims.push_back(ext[lab]);
+          }
+          if (!sw) { c.a_kfast = lk; c.b_kfast = rk; }
+          else { c.a_kfast = rk; c.b_kfast = lk; }
+          c.bytes_moved = cost;
+          cands.push_back(c);
+        }
+  // deterministic preference on ties: fewer bytes, then k-fast operands, then no swap
+  auto score = [](const ContractPlan &c) {
+    return c.bytes_moved * 4.0 + (c.a_kfast ? 0 : 1) + (c.b_kfast ? 0 : 1) + (c.swap ? 0.5 : 0);
+  };
+  std::stable_sort(cands.begin(), cands.end(),
+                   [&](const ContractPlan &a, const ContractPlan &b) { return score(a) < score(b); });
+  // orders that coincide (e.g. L's free indices already in D's order) give
+  // the same launches: keep the first of each
+  std::vector<ContractPlan> uniq;
+  for (const ContractPlan &c : cands) {
+    bool dup = false;
+    for (const ContractPlan &u : uniq)
+      if (u.swap == c.swap && u.a_kfast == c.a_kfast && u.b_kfast == c.b_kfast && u.l_o2n == c.l_o2n &&
+          u.r_o2n == c.r_o2n && u.d_in_o2n == c.d_in_o2n && u.m == c.m && u.n == c.n && u.k == c.k) {
+        dup = true;
+        break;
+      }
+    if (!dup) uniq.push_back(c);
+  }
+  cands.swap(uniq);
+  if (all) *all = cands;
+  if (variant >= 0 && variant < (int)cands.size()) P = cands[variant];
+  else P = cands.front();
+  return 0;
+}
+
+
+// ---- features ------------------------------------------------------------------
+namespace {
+
+inline float lg2(double v) { return (float)std::log2(v < 1.0 ? 1.0 : v); }
+
+// Shape of a DLF permutation after dropping unit extents and fusing index runs
+// that stay adjacent: fused rank, contiguous input run and contiguous output
+// run (elements).
+void perm_shape(const std::vector<int> &dims, const std::vector<int> &o2n, double *in_run, double *out_run,
+                int *frank)
+{
+  std::vector<int> d, p;
+  for (size_t i = 0; i < dims.size(); ++i)
+    if (dims[i] > 1) { d.push_back(dims[i]); p.push_back(o2n[i]); }
+  const int r = (int)d.size();
+  std::vector<int> idx(r), rank(r);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::sort(idx.begin(), idx.end(), [&](int a, int b) { return p[a] < p[b]; });
+  for (int q = 0; q < r; ++q) rank[idx[q]] = q;
+  std::vector<std::pair<double, int>> groups;  // (extent, first output rank)
+  for (int i = 0; i < r; ++i) {
+    if (i > 0 && rank[i] == rank[i - 1] + 1) groups.back().first *= d[i];
+    else groups.push_back({(double)d[i], rank[i]});
+  }
+  *frank = (int)groups.size();
+  *in_run = groups.empty() ? 1.0 : groups[0].first;
+  *out_run = 1.0;
+  for (auto &g : groups)
+    if (g.second == 0) *out_run = g.first;
+}
+
+void put_perm(float *f, const std::vector<int> &dims, const std::vector<int> &o2n, double bytes)
+{
+  if (o2n.empty()) return;  // no transpose: all zero
+  double ir, orr;
+  int fr;
+  perm_shape(dims, o2n, &ir, &orr, &fr);
+  f[0] = 1.f;
+  f[1] = lg2(bytes);
+  f[2] = lg2(ir);
+  f[3] = lg2(orr);
+  f[4] = (float)fr;
+}
+
+}  // namespace
+
+int plan_candidates(const std::vector<int> &dig, int dr, int lr, int rr, const std::vector<int> &dd,
+                    const std::vector<int> &ld, const std::vector<int> &rd, int kind, R4Mode r4mode, bool acc,
+                    int num_cus, std::vector<PlanCandidate> &out)
+{
+  out.clear();
+  const int es = kind_size(kind);
+  if (!es) return TALSH_INVALID_ARGS;
+  ContractPlan P;
+  std::vector<ContractPlan> all;
+  const int e = plan_contraction(dig, dr, lr, rr, dd, ld, rd, es, acc, -1, P, &all);
+  if (e) return e;
+  size_t vl = 1, vr = 1, vd = 1;
+  for (int x : ld) vl *= (size_t)x;
+  for (int x : rd) vr *= (size_t)x;
+  for (int x : dd) vd *= (size_t)x;
+  const bool cplx = kind_complex(kind);
+  const int ntiles = gemm_tile_choices(kind, r4mode);
+  for (int v = 0; v < (int)all.size(); ++v) {
+    const ContractPlan &c = all[v];
+    const double gm = (double)(c.swap ? c.n : c.m), gn = (double)(c.swap ? c.m : c.n), gk = (double)c.k;
+    for (int t = 0; t < ntiles; ++t) {
+      PlanCandidate pc;
+      pc.variant = v;
+      pc.tile = t;
+      float *f = pc.f;
+      f[0] = kind == R4; f[1] = kind == R8; f[2] = kind == C4; f[3] = kind == C8;
+      f[4] = kind == R4 && r4mode == R4Mode::kBF16x3;
+      f[5] = acc;
+      f[6] = lg2(gm); f[7] = lg2(gn); f[8] = lg2(gk);
+      f[9] = lg2((cplx ? 8.0 : 2.0) * gm * gn * gk);
+      f[10] = c.a_kfast; f[11] = c.b_kfast; f[12] = c.swap;
+      f[13 + t] = 1.f;  // tile one-hot (3 slots)
+      f[16] = lg2(1.0 + c.bytes_moved);
+      f[17] = lg2((double)es * (gm * gk + gn * gk + gm * gn * (acc ? 2.0 : 1.0)));
+      put_perm(f + 18, ld, c.l_o2n, 2.0 * vl * es);
+      put_perm(f + 23, rd, c.r_o2n, 2.0 * vr * es);
+      if (!c.d_out_o2n.empty()) put_perm(f + 28, c.dmat_dims, c.d_out_o2n, (acc ? 4.0 : 2.0) * vd * es);
+      int bm, bn, occ;
+      gemm_tile_shape(kind, r4mode, t, &bm, &bn, &occ);
+      const double tm = std::ceil(gm / bm), tn = std::ceil(gn / bn);
+      f[33] = lg2(tm * bm * tn * bn / (gm * gn));
+      f[34] = (float)std::log2(tm * tn / (double)(num_cus > 0 ? num_cus : 256));
+      f[35] = (float)occ;
+      out.push_back(pc);
+    }
+  }
+  return 0;
+}
+
+// ---- host inference and the installed selector --------------------------------
+float mlp_forward_host(const MlpShape &s, const float *params, const float *x)
+{
+  float a[kMlpMaxWidth], b[kMlpMaxWidth];
+  std::memcpy(a, x, sizeof(float) * s.nin);
+  for (int l = 0; l <= s.hidden; ++l) {
+    size_t wo, bo;
+    int ni, no;
+    mlp_layer_offsets(s, l, &wo, &bo, &ni, &no);
+    const float *W = params + wo, *bias = params + bo;
+    const bool last = l == s.hidden;
+    for (int o = 0; o < no; ++o) {
+      float acc = bias[o];
+      for (int i = 0; i < ni; ++i) acc = std::fmaf(a[i], W[i * no + o], acc);
+      b[o] = last ? acc : std::max(acc, 0.f);
+    }
+    std::memcpy(a, b, sizeof(float) * no);
+  }
+  return a[0];
+}
+
+float MlpModel::predict_host(const float *x) const
+{
+  float xn[kMlpMaxWidth];
+  normalize(x, xn);
+  return mlp_forward_host(s, params.data(), xn) * ystd + ymean;
+}
+
+namespace {
+std::mutex g_sel_mu;
+MlpModel *g_sel = nullptr;
+std::unordered_map<std::string, std::pair<int, int>> g_sel_memo;
+}  // namespace
+
+void selector_install(const MlpModel *m)
+{
+  std::lock_guard<std::mutex> g(g_sel_mu);
+  delete g_sel;
+  g_sel = m ? new MlpModel(*m) : nullptr;
+  g_sel_memo.clear();
+}
+
+bool selector_installed()
+{
+  std::lock_guard<std::mutex> g(g_sel_mu);
+  return g_sel != nullptr;
+}
+
+bool selector_choose(const std::vector<int> &dig, int dr, int lr, int rr, const std::vector<int> &dd,
+                     const std::vector<int> &ld, const std::vector<int> &rd, int kind, R4Mode r4mode, bool acc,
+                     int num_cus, int *variant, int *tile)
+{
+  std::lock_guard<std::mutex> g(g_sel_mu);
+  if (!g_sel) return false;
+  std::string key;
+  key.reserve(64);
+  auto put = [&](int v) { key.append(std::to_string(v)).push_back(','); };
+  put(kind); put((int)r4mode); put(acc); put(lr); put(rr); put(num_cus);
+  for (int v : dig) put(v);
+  for (int v : dd) put(v);
+  for (int v : ld) put(v);
+  for (int v : rd) put(v);
+  auto it = g_sel_memo.find(key);
+  if (it != g_sel_memo.end()) {
+    *variant = it->second.first;
+    *tile = it->second.second;
+    return true;
+  }
+  std::vector<PlanCandidate> cands;
+  if (plan_candidates(dig, dr, lr, rr, dd, ld, rd, kind, r4mode, acc, num_cus, cands) || cands.empty() ||
+      g_sel->s.nin != kPlanFeatures)
+    return false;
+  float best = 0.f;
+  int bi = -1;
+  for (int i = 0; i < (int)cands.size(); ++i) {
+    const float y = g_sel->predict_host(cands[i].f);
+    if (bi < 0 || y < best) { best = y; bi = i; }
+  }
+  *variant = cands[bi].variant;
+  *tile = cands[bi].tile;
+  g_sel_memo[key] = {*variant, *tile};
+  return true;
+}
+
+}  // namespace xt

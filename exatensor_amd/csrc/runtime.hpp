@@ -85,6 +85,12 @@ struct Task {
 
 int task_finalize(talsh_task_t *t, int status);
 
+// Pooled HIP events of the current device (timing or not); put returns one
+// for reuse once nothing will query it again.
+hipEvent_t event_get(bool timing);
+void event_put(hipEvent_t e, bool timing);
+void event_pool_clear();
+
 // ---------------------------------------------------------------------------
 // Profiling of the dominant kernels (event pairs on the launch stream).
 void prof_begin(hipStream_t s, int kernel_id, hipEvent_t *pair);

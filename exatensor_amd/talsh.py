@@ -63,6 +63,7 @@ class TalshTask(C.Structure):
 
 
 _IP = C.POINTER(C.c_int)
+_FP = C.POINTER(C.c_float)
 _TP = C.POINTER(TalshTens)
 _KP = C.POINTER(TalshTask)
 _lib = None
@@ -134,6 +135,24 @@ def load_library(path=LIB_PATH):
         "talshAmdBatchExecute": (C.c_int, [C.c_void_p, C.c_int]),
         "talshAmdBatchInfo": (C.c_int, [C.c_void_p, _IP, _IP, C.POINTER(C.c_double)]),
         "talshAmdBatchDestroy": (C.c_int, [C.c_void_p]),
+        "talshAmdMlpCreate": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]),
+        "talshAmdMlpDestroy": (C.c_int, [C.c_void_p]),
+        "talshAmdMlpNumParams": (C.c_int, [C.c_void_p, _IP]),
+        "talshAmdMlpGetParams": (C.c_int, [C.c_void_p, _FP]),
+        "talshAmdMlpSetParams": (C.c_int, [C.c_void_p, _FP]),
+        "talshAmdMlpSetNormalization": (C.c_int, [C.c_void_p, _FP, _FP, C.c_float, C.c_float]),
+        "talshAmdMlpGradient": (C.c_int, [C.c_void_p, _FP, _FP, C.c_int, _FP, _FP]),
+        "talshAmdMlpPredict": (C.c_int, [C.c_void_p, _FP, C.c_longlong, _FP]),
+        "talshAmdMlpTrain": (C.c_int, [C.c_void_p, _FP, _FP, C.c_longlong, C.c_int, C.c_int, C.c_float,
+                                       C.c_ulonglong, _FP]),
+        "talshAmdMlpSave": (C.c_int, [C.c_void_p, C.c_char_p]),
+        "talshAmdMlpLoad": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+        "talshAmdPlanFeatureCount": (C.c_int, []),
+        "talshAmdPlanFeatures": (C.c_int, [C.c_char_p, C.c_int, _IP, C.c_int, _IP, C.c_int, _IP, C.c_int, C.c_int,
+                                           C.c_int, C.c_int, _IP, _IP, _FP, _IP]),
+        "talshAmdSelectorInstall": (C.c_int, [C.c_void_p]),
+        "talshAmdSelectorChoice": (C.c_int, [C.c_char_p, C.c_int, _IP, C.c_int, _IP, C.c_int, _IP, C.c_int, C.c_int,
+                                             C.c_int, _IP, _IP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -66,6 +66,48 @@ int talshAmdBatchExecute(void *batch, int use_graph);
 int talshAmdBatchInfo(void *batch, int *nops, int *nitems, double *flops);
 int talshAmdBatchDestroy(void *batch);
 
+/* ---- learned contraction-plan selector ----
+ * A multilayer perceptron (nin inputs, 1-3 hidden ReLU layers of width <= 64,
+ * one linear output) that predicts the log of a contraction candidate's
+ * kernel time from its features; trained on the GPU with the engine's own
+ * forward/backward/Adam kernels. Parameters are FP32, stored layer by layer
+ * as W[in][out] (out fastest) followed by b[out]. */
+int talshAmdMlpCreate(int nin, int hidden_layers, int width, unsigned long long seed, void **mlp);
+int talshAmdMlpDestroy(void *mlp);
+int talshAmdMlpNumParams(void *mlp, int *nparams);
+int talshAmdMlpGetParams(void *mlp, float *params);
+int talshAmdMlpSetParams(void *mlp, const float *params);
+/* Inputs are normalized as (x - x_mean) / x_std, outputs de-normalized as
+ * y * y_std + y_mean (training sets both from its data). */
+int talshAmdMlpSetNormalization(void *mlp, const float *x_mean, const float *x_std, float y_mean, float y_std);
+/* Mean squared error (in normalized output units) over n rows and its
+ * gradient with respect to every parameter, computed on the GPU. */
+int talshAmdMlpGradient(void *mlp, const float *x, const float *y, int n, float *grad, float *loss);
+/* GPU forward of n rows of raw features into y[n]. */
+int talshAmdMlpPredict(void *mlp, const float *x, long long n, float *y);
+/* Adam training on the GPU over n rows (x[n][nin], y[n]); epoch_loss[epochs]
+ * receives the mean training loss of every epoch (may be NULL). */
+int talshAmdMlpTrain(void *mlp, const float *x, const float *y, long long n, int epochs, int batch, float lr,
+                     unsigned long long seed, float *epoch_loss);
+int talshAmdMlpSave(void *mlp, const char *path);
+int talshAmdMlpLoad(const char *path, void **mlp);
+/* Features of every (matricization variant, GEMM tile) candidate of a
+ * contraction: features[ncand][talshAmdPlanFeatureCount()], candidate i runs
+ * with talshAmdSetPlanVariant(variants[i], tiles[i]). */
+int talshAmdPlanFeatureCount(void);
+int talshAmdPlanFeatures(const char *cptrn, int drank, const int *ddims, int lrank, const int *ldims, int rrank,
+                         const int *rdims, int data_kind, int accumulative, int fast_math, int cap, int *variants,
+                         int *tiles, float *features, int *ncand);
+/* Install a copy of a trained model as the planner of talshTensorContract
+ * (NULL removes it; without a model the byte-count heuristic plans). The
+ * environment variable TALSH_PLAN_MODEL names a model file loaded at
+ * talshInit. */
+int talshAmdSelectorInstall(void *mlp);
+/* The candidate the installed selector picks (variant 0, tile -1: the
+ * heuristic's choice when no model is installed). */
+int talshAmdSelectorChoice(const char *cptrn, int drank, const int *ddims, int lrank, const int *ldims, int rrank,
+                           const int *rdims, int data_kind, int accumulative, int fast_math, int *variant, int *tile);
+
 /* Synchronize all engine streams of a GPU (kind-specific number). */
 int talshAmdDeviceSynchronize(int dev_num);
 

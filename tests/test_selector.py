@@ -1,0 +1,226 @@
+"""Plan selector: generator, candidate features, model persistence and the
+host-side choice (CPU); MLP gradients, GPU forward, training and the planner
+with a model installed (GPU)."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import mlp_ref
+from exatensor_amd import selector as S
+from exatensor_amd import talsh as T
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return T.load_library()
+
+
+def test_generator_is_seeded_and_in_range(lib):
+    a = [S.random_contraction(np.random.default_rng(5)) for _ in range(3)]
+    b = [S.random_contraction(np.random.default_rng(5)) for _ in range(3)]
+    assert a == b
+    rng = np.random.default_rng(9)
+    for _ in range(200):
+        s = S.random_contraction(rng)
+        for key in ("ddims", "ldims", "rdims"):
+            assert 2 <= len(s[key]) <= 6
+        vmax = max(np.prod(s[k], dtype=float) for k in ("ddims", "ldims", "rdims"))
+        assert 2 ** 15.5 <= vmax <= 2 ** 26.5
+        assert T.str2dig(lib, s["pattern"])[0] == 0
+
+
+def test_candidate_features(lib):
+    rng = np.random.default_rng(3)
+    nf = lib.talshAmdPlanFeatureCount()
+    for _ in range(50):
+        s = S.random_contraction(rng)
+        v, t, f = S.plan_features(lib, s)
+        assert f.shape == (len(v), nf) and np.all(np.isfinite(f))
+        ntiles = 3 if s["kind"] in (T.R4, T.R8) else 1
+        assert len(v) % ntiles == 0
+        assert set(t.tolist()) == set(range(ntiles))
+        # one-hot data kind and tile
+        assert np.all(f[:, 0:4].sum(1) == 1) and np.all(f[:, 13:16].sum(1) == 1)
+        # variant 0 is the heuristic's choice: fewest transpose bytes
+        assert f[v == 0, 16].min() <= f[:, 16].min() + 1e-6
+        moved = (C.c_double * 64)()
+        nc = C.c_int()
+        e = lib.talshAmdPlanCandidates(s["pattern"].encode(), len(s["ddims"]), T.ints(s["ddims"]), len(s["ldims"]),
+                                       T.ints(s["ldims"]), len(s["rdims"]), T.ints(s["rdims"]), s["kind"],
+                                       T.YEP if s["acc"] else T.NOPE, 64, moved, C.byref(nc))
+        assert e == 0 and nc.value == len(set(v.tolist()))
+
+
+def test_model_file_roundtrip_and_host_choice(lib, tmp_path):
+    m = S.Mlp(lib, lib.talshAmdPlanFeatureCount(), 2, 32, seed=11)
+    p = m.params()
+    rng = np.random.default_rng(0)
+    mean = rng.normal(size=36).astype(np.float32)
+    std = rng.uniform(0.5, 2, size=36).astype(np.float32)
+    assert lib.talshAmdMlpSetNormalization(m.h, mean.ctypes.data_as(T._FP), std.ctypes.data_as(T._FP), 1.5, 0.5) == 0
+    path = str(tmp_path / "m.txt")
+    m.save(path)
+    m2 = S.Mlp(lib, path=path)
+    assert np.array_equal(m2.params(), p)
+    # the installed model picks the candidate the float64 restatement ranks first
+    assert m2.install() == 0
+    try:
+        for _ in range(30):
+            s = S.random_contraction(rng)
+            v, t, f = S.plan_features(lib, s)
+            xn = (f.astype(np.float64) - mean) / std
+            pred, _ = mlp_ref.forward(p.astype(np.float64), xn, 36, 2, 32)
+            cv, ct = C.c_int(), C.c_int()
+            assert lib.talshAmdSelectorChoice(s["pattern"].encode(), len(s["ddims"]), T.ints(s["ddims"]),
+                                              len(s["ldims"]), T.ints(s["ldims"]), len(s["rdims"]),
+                                              T.ints(s["rdims"]), s["kind"], T.YEP if s["acc"] else T.NOPE, 0,
+                                              C.byref(cv), C.byref(ct)) == 0
+            i = [k for k in range(len(v)) if v[k] == cv.value and t[k] == ct.value][0]
+            assert pred[i] <= pred.min() + 1e-4 * max(1.0, abs(pred.min()))
+    finally:
+        lib.talshAmdSelectorInstall(None)
+        m.close()
+        m2.close()
+
+
+def test_numpy_reference_gradient_is_exact():
+    """Pins the float64 restatement itself by central finite differences."""
+    rng = np.random.default_rng(1)
+    nin, hid, wid = 7, 2, 5
+    _, n = mlp_ref.layers(nin, hid, wid)
+    p = rng.normal(size=n) * 0.5
+    x = rng.normal(size=(13, nin))
+    y = rng.normal(size=13)
+    _, g = mlp_ref.loss_and_grad(p, x, y, nin, hid, wid)
+    for i in rng.choice(n, 25, replace=False):
+        e = np.zeros(n)
+        e[i] = 1e-6
+        fd = (mlp_ref.loss_and_grad(p + e, x, y, nin, hid, wid)[0] -
+              mlp_ref.loss_and_grad(p - e, x, y, nin, hid, wid)[0]) / 2e-6
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i]))
+
+
+def test_committed_model_loads(lib):
+    if not os.path.exists(S.DEFAULT_MODEL):
+        pytest.skip("no trained model committed")
+    m = S.Mlp(lib, path=S.DEFAULT_MODEL)
+    assert m.nparams > 0 and np.all(np.isfinite(m.params()))
+    m.close()
+
+
+# ---------------------------------------------------------------------------- GPU
+@pytest.fixture(scope="module")
+def eng():
+    from gpu_util import engine
+    return engine()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("hidden", [1, 2, 3])
+def test_mlp_gradient_matches_cpu(eng, hidden):
+    lib = eng.lib
+    nin, wid, n = 36, 64, 300
+    m = S.Mlp(lib, nin, hidden, wid, seed=hidden)
+    rng = np.random.default_rng(hidden)
+    x = rng.normal(size=(n, nin)).astype(np.float32)
+    y = rng.normal(size=n).astype(np.float32)
+    g, loss = m.gradient(x, y)
+    l64, g64 = mlp_ref.loss_and_grad(m.params().astype(np.float64), x.astype(np.float64), y.astype(np.float64), nin,
+                                     hidden, wid)
+    assert abs(loss - l64) <= 1e-5 * l64
+    assert np.linalg.norm(g - g64) <= 1e-5 * np.linalg.norm(g64)
+    m.close()
+
+
+@pytest.mark.gpu
+def test_gpu_forward_matches_host_planner(eng):
+    lib = eng.lib
+    m = S.Mlp(lib, 36, 2, 64, seed=4)
+    rng = np.random.default_rng(4)
+    mean = rng.normal(size=36).astype(np.float32)
+    std = rng.uniform(0.5, 2, size=36).astype(np.float32)
+    lib.talshAmdMlpSetNormalization(m.h, mean.ctypes.data_as(T._FP), std.ctypes.data_as(T._FP), -2.0, 3.0)
+    m.install()
+    try:
+        for _ in range(40):
+            s = S.random_contraction(rng)
+            v, t, f = S.plan_features(lib, s)
+            pred = m.predict(f)
+            ref, _ = mlp_ref.forward(m.params().astype(np.float64), (f.astype(np.float64) - mean) / std, 36, 2, 64)
+            assert np.allclose(pred, ref * 3.0 - 2.0, rtol=1e-5, atol=1e-5)
+            cv, ct = C.c_int(), C.c_int()
+            lib.talshAmdSelectorChoice(s["pattern"].encode(), len(s["ddims"]), T.ints(s["ddims"]), len(s["ldims"]),
+                                       T.ints(s["ldims"]), len(s["rdims"]), T.ints(s["rdims"]), s["kind"],
+                                       T.YEP if s["acc"] else T.NOPE, 0, C.byref(cv), C.byref(ct))
+            i = int(np.argmin(pred))
+            # host inference runs the same fused multiply-adds in the same order
+            assert (v[i], t[i]) == (cv.value, ct.value)
+    finally:
+        lib.talshAmdSelectorInstall(None)
+        m.close()
+
+
+@pytest.mark.gpu
+def test_training_fits_a_smooth_function(eng):
+    lib = eng.lib
+    rng = np.random.default_rng(8)
+    x = rng.normal(size=(20000, 36)).astype(np.float32)
+    y = (np.sin(x[:, 0]) + 0.5 * x[:, 1] * x[:, 2] + np.abs(x[:, 3])).astype(np.float32)
+    m = S.Mlp(lib, 36, 2, 64, seed=8)
+    loss = m.train(x, y, epochs=25, batch=256, lr=2e-3)
+    assert np.all(np.isfinite(loss)) and loss[-1] < 0.1 * loss[0]
+    pred = m.predict(x[:2000])
+    assert np.sqrt(np.mean((pred - y[:2000]) ** 2)) < 0.3 * np.std(y)
+    m.close()
+
+
+@pytest.mark.gpu
+def test_planner_with_model_gives_same_results(eng, oracle):
+    """With a (random) model installed the planner runs other matricizations
+    and tile shapes; results must still match the oracle."""
+    from gpu_util import contract, rel_err
+    lib = eng.lib
+    m = S.Mlp(lib, 36, 2, 64, seed=21)
+    m.install()
+    try:
+        rng = np.random.default_rng(21)
+        n = 0
+        while n < 6:
+            s = S.random_contraction(rng, log2_vol=(8, 12))
+            kind = s["kind"]
+            vol = lambda k: int(np.prod(s[k]))
+            if kind == T.C8:
+                l, r, d = (oracle.normal(T.C8, vol(k), 40 + i) for i, k in enumerate(("ldims", "rdims", "ddims")))
+            else:
+                kind = T.R8
+                l, r, d = (oracle.uniform(T.R8, [vol(k)], 40 + i) for i, k in enumerate(("ldims", "rdims", "ddims")))
+            ex = oracle.contract_exact(s["pattern"], d, s["ddims"], l, s["ldims"], r, s["rdims"], 0.5, s["acc"])
+            e, got = contract(s["pattern"], d, s["ddims"], l, s["ldims"], r, s["rdims"], 0.5, s["acc"])
+            assert e == 0
+            assert rel_err(got, ex) <= 1e-12, s
+            n += 1
+    finally:
+        lib.talshAmdSelectorInstall(None)
+        m.close()
+
+
+@pytest.mark.gpu
+def test_collect_train_evaluate_pipeline(eng):
+    lib = eng.lib
+    rng = np.random.default_rng(30)
+    specs, times = [], []
+    for _ in range(12):
+        s = S.random_contraction(rng, log2_vol=(12, 16))
+        _, _, f, tm = S.time_candidates(lib, eng.gpu(0), s, reps=1)
+        assert len(tm) == len(f) and np.all(tm > 0)
+        specs.append(s)
+        times.append(tm)
+    X, y, groups = S.design_matrix(lib, specs, times)
+    m = S.Mlp(lib, X.shape[1], 2, 64, seed=1)
+    m.train(X, y, epochs=40, batch=64, lr=3e-3)
+    rep = S.regret_report(lib, specs, times, m)
+    lib.talshAmdSelectorInstall(None)
+    assert rep["contractions"] == 12 and rep["mean_regret"] >= 0 and rep["heuristic_mean_regret"] >= 0
+    m.close()
