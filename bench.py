@@ -9,7 +9,10 @@ Algorithmic bytes per transpose = 2 * volume * element size (read + write).
 Also reported (not the headline value): dense contractions through
 talshTensorContract (R8 on FP64 matrix cores, R4 native FP32, R4 three-way
 BF16 split, C8) on D(a,b,i,j)+=L(a,k,i,l)*R(l,b,k,j) with a=b=i=j=96, k=l=128,
-and D(a,b,c,d)+=L(d,i,b,j)*R(j,c,i,a) with extents 32.
+and D(a,b,c,d)+=L(d,i,b,j)*R(j,c,i,a) with extents 32; a batch of 4096
+block contractions D(a,b,c,d)+=L(a,k,c,l)*R(l,b,k,d) (one grouped launch vs
+one asynchronous talshTensorContract task per block); the learned plan
+selector's regret against exhaustive timing on fresh contractions.
 
 Prints ONE JSON line.
 """
@@ -314,6 +317,30 @@ def run_blocksparse(gb, reps=3):
     return out
 
 
+def run_selector(gb, n=300, seed=303):
+    """C5: regret of the committed plan-selector model on n fresh seeded
+    contractions (disjoint seed from training), against exhaustive timing of
+    every candidate, next to the byte-count heuristic's regret."""
+    from exatensor_amd import selector as S
+    if not os.path.exists(S.DEFAULT_MODEL):
+        return {"skipped": "no trained model"}
+    rng = np.random.default_rng(seed)
+    specs, times = [], []
+    t0 = time.perf_counter()
+    for _ in range(n):
+        spec = S.random_contraction(rng)
+        _, _, _, tm = S.time_candidates(gb.L, gb.gpu, spec)
+        specs.append(spec)
+        times.append(tm)
+    m = S.Mlp(gb.L, path=S.DEFAULT_MODEL)
+    rep = S.regret_report(gb.L, specs, times, m)
+    gb.L.talshAmdSelectorInstall(None)
+    m.close()
+    rep["seed"] = seed
+    rep["timing_seconds"] = round(time.perf_counter() - t0, 1)
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in rep.items()}
+
+
 def cpu_baseline(target_seconds=12.0):
     """The reference TAL-SH CPU path (oracle/_ref, built from the reference's
     own sources) timed on this host's cores for a bounded sample of the same
@@ -385,6 +412,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-contract", action="store_true", help="skip the contraction components")
     ap.add_argument("--no-blocksparse", action="store_true", help="skip the block-sparse component")
+    ap.add_argument("--no-selector", action="store_true", help="skip the plan-selector regret component")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, run identity copies of known size (counter calibration)")
     args = ap.parse_args()
@@ -399,6 +427,8 @@ def main():
     comps = {} if args.no_contract else run_contractions(gb)
     if not args.no_blocksparse:
         comps["blocksparse"] = run_blocksparse(gb)
+    if not args.no_selector:
+        comps["plan_selector"] = run_selector(gb)
     value = tr["bytes"] / 1e9 / tr["seconds"]
     achieved = tr["achieved_gbs"]
     line = {

@@ -313,6 +313,7 @@ bool selector_choose(const std::vector<int> &dig, int dr, int lr, int rr, const 
   }
   *variant = cands[bi].variant;
   *tile = cands[bi].tile;
+  if (g_sel_memo.size() >= 65536) g_sel_memo.clear();
   g_sel_memo[key] = {*variant, *tile};
   return true;
 }
