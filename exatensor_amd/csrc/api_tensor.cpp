@@ -3,11 +3,14 @@
 // Behaviour follows TALSH/talshc.cpp (init :546-621, construct :1167-1302,
 // tensor queries :1304-1700, tasks :1935-2597); the runtime underneath is this
 // engine's (arena per GPU, stream pool, per-image ordering events).
+#include <dlfcn.h>
+
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <string>
 #include <unordered_map>
 
 #include <algorithm>
@@ -366,12 +369,29 @@ int talshInit(size_t *host_buf_size, int *host_arg_max, int ngpus, int gpu_list[
   if (host_arg_max) *host_arg_max = 1 << 20;
   (void)host_buf_size;  // host images are allocated per tensor (pinned when a GPU is present)
   E.on = true;
-  if (const char *pm = std::getenv("TALSH_PLAN_MODEL")) {  // learned plan selector
+  {  // learned plan selector: TALSH_PLAN_MODEL, else the model shipped beside the library
+    std::string path;
+    const char *pm = std::getenv("TALSH_PLAN_MODEL");
+    bool must = false;
+    if (pm) {
+      path = pm;
+      must = !path.empty();
+    } else {
+      Dl_info info;
+      if (dladdr(reinterpret_cast<void *>(&talshInit), &info) && info.dli_fname) {
+        path = info.dli_fname;
+        const size_t slash = path.rfind('/');
+        path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/plan_model.txt";
+        FILE *f = std::fopen(path.c_str(), "r");
+        if (f) std::fclose(f);
+        else path.clear();
+      }
+    }
     void *h = nullptr;
-    if (pm[0] && talshAmdMlpLoad(pm, &h) == 0) {
+    if (!path.empty() && talshAmdMlpLoad(path.c_str(), &h) == 0) {
       talshAmdSelectorInstall(h);
       talshAmdMlpDestroy(h);
-    } else if (pm[0]) {
+    } else if (must) {
       log_error("TALSH_PLAN_MODEL=%s could not be loaded; contractions use the byte-count heuristic", pm);
     }
   }

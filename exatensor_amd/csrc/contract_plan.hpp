@@ -47,8 +47,15 @@ int plan_candidates(const std::vector<int> &dig, int dr, int lr, int rr, const s
                     const std::vector<int> &ld, const std::vector<int> &rd, int kind, R4Mode r4mode, bool acc,
                     int num_cus, std::vector<PlanCandidate> &out);
 
+// Training domain of the selector (random contractions up to 2^35 flops and
+// operands of up to 2^26 elements, with margin); larger problems use the
+// heuristic.
+constexpr double kSelectorMaxLog2Flops = 35.5;
+constexpr double kSelectorMaxLog2Volume = 26.5;
+
 // If a selector model is installed: its choice of (variant, tile) for this
-// contraction (memoized per shape) and true; false otherwise.
+// contraction (memoized per shape) and true; false otherwise (no model, or
+// the contraction lies outside the model's training domain).
 bool selector_choose(const std::vector<int> &dig, int dr, int lr, int rr, const std::vector<int> &dd,
                      const std::vector<int> &ld, const std::vector<int> &rd, int kind, R4Mode r4mode, bool acc,
                      int num_cus, int *variant, int *tile);

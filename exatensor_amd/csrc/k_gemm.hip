@@ -470,6 +470,141 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const B3Params p)
     }
 }
 
+// Second form of the three-way split kernel: a 256x128 tile per workgroup of
+// 8 waves (2 per SIMD, each wave 64x64 of 16x16x32 MFMAs), operands moved
+// from global memory straight into LDS (global_load_lds_dwordx4, no register
+// staging) into a double-buffered stage, and split K: gridDim.y chunks of at
+// most kB3Chunk k-values; inside a chunk the FP32 MFMA accumulator is folded
+// every kFold k-tiles into a second FP32 sum (two-level summation), and each
+// chunk's partial tile goes to a workspace that k_splitk_reduce sums in FP64.
+//
+// LDS rows hold 32 bf16 (four 16-byte chunks); chunk c of row r sits at
+// position c ^ ((r >> 2) & 3), so the 16 rows a fragment read touches at one
+// chunk index land on distinct banks.
+constexpr int kB3Chunk = 2048;
+constexpr int kB3M = 256, kB3N = 128;
+struct B3Split {
+  B3Params p;
+  float *part;       // ksplit partial C tiles, m x n column-major each (ldc = m)
+  int64_t kchunk;    // k-values per chunk (multiple of 32)
+};
+
+__device__ __forceinline__ void lds_dma16(const void *g, void *lds_wave_base)
+{
+  __builtin_amdgcn_global_load_lds(g, lds_wave_base, 16, 0, 0);
+}
+
+__global__ __launch_bounds__(512, 1) void k_gemm_bf16x3v3(const B3Split q)
+{
+  constexpr int BM = kB3M, BN = kB3N, BK = 32;
+  constexpr int FM = 4, FN = 4;          // 16x16 blocks per wave (64x64)
+  constexpr int kFold = 8;
+  constexpr int LA = 3 * (BM / 16) / 8;  // wave loads of A per stage (16 rows x 64 B each)
+  constexpr int LB = 3 * (BN / 16) / 8;
+  const B3Params &p = q.p;
+  __shared__ __align__(16) unsigned short As[2][3][BM][BK];
+  __shared__ __align__(16) unsigned short Bs[2][3][BN][BK];
+  int tm, tn;
+  tile_of_b3(p, tm, tn);
+  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+  const int64_t kbeg = (int64_t)blockIdx.y * q.kchunk;
+  const int64_t kend = min(kbeg + q.kchunk, p.k_pad);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 3) * 64, wn = (wave >> 2) * 64;
+  const int64_t pa = p.a_rows_pad * p.k_pad, pb = p.b_rows_pad * p.k_pad;
+  // this lane's slot in a wave load: row (lane / 4) of 16, chunk position lane % 4
+  const int lrow = lane >> 2, lpos = lane & 3;
+  const int lchunk = lpos ^ ((lrow >> 2) & 3);
+  auto issue = [&](int buf, int64_t k0) {
+#pragma unroll
+    for (int u = 0; u < LA; ++u) {
+      const int idx = wave * LA + u, pl = idx / (BM / 16), rg = idx % (BM / 16);
+      const unsigned short *g = p.a3 + pl * pa + (m0 + rg * 16 + lrow) * p.k_pad + k0 + lchunk * 8;
+      lds_dma16(g, &As[buf][pl][rg * 16][0]);
+    }
+#pragma unroll
+    for (int u = 0; u < LB; ++u) {
+      const int idx = wave * LB + u, pl = idx / (BN / 16), rg = idx % (BN / 16);
+      const unsigned short *g = p.b3 + pl * pb + (n0 + rg * 16 + lrow) * p.k_pad + k0 + lchunk * 8;
+      lds_dma16(g, &Bs[buf][pl][rg * 16][0]);
+    }
+  };
+  f4 acc[FM][FN], sum[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      acc[i][j] = f4{0, 0, 0, 0};
+      sum[i][j] = f4{0, 0, 0, 0};
+    }
+  const int64_t nk = (kend - kbeg) / BK;
+  const int fr = lane & 15, fc = lane >> 4;
+  const int fpos = (fc ^ ((fr >> 2) & 3)) * 8;  // swizzled chunk of this lane's fragment
+  issue(0, kbeg);
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's loads of stage kt have landed in LDS
+    __syncthreads();                // everyone's have; stage kt^1 is free to refill
+    if (kt + 1 < nk) issue(cur ^ 1, kbeg + (kt + 1) * BK);
+    us8 fb[3][FN];
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) fb[pl][j] = *reinterpret_cast<const us8 *>(&Bs[cur][pl][wn + j * 16 + fr][fpos]);
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      us8 fa[3];
+#pragma unroll
+      for (int pl = 0; pl < 3; ++pl) fa[pl] = *reinterpret_cast<const us8 *>(&As[cur][pl][wm + i * 16 + fr][fpos]);
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        acc[i][j] = mfma_bf(fa[0], fb[2][j], acc[i][j]);
+        acc[i][j] = mfma_bf(fa[1], fb[1][j], acc[i][j]);
+        acc[i][j] = mfma_bf(fa[2], fb[0][j], acc[i][j]);
+        acc[i][j] = mfma_bf(fa[0], fb[1][j], acc[i][j]);
+        acc[i][j] = mfma_bf(fa[1], fb[0][j], acc[i][j]);
+        acc[i][j] = mfma_bf(fa[0], fb[0][j], acc[i][j]);
+      }
+    }
+    if ((kt + 1) % kFold == 0 || kt + 1 == nk) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          sum[i][j] += acc[i][j];
+          acc[i][j] = f4{0, 0, 0, 0};
+        }
+    }
+  }
+  float *P = q.part + (int64_t)blockIdx.y * p.m * p.n;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int64_t col = n0 + wn + j * 16 + fr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+        if (row < p.m && col < p.n) P[row + col * p.m] = sum[i][j][r];
+      }
+    }
+}
+
+// C = alpha * sum_z part[z] (+ C): FP64 sum over the K chunks.
+__global__ __launch_bounds__(256) void k_splitk_reduce(const float *__restrict__ part, int nz, int64_t mn, int64_t m,
+                                                       float *__restrict__ c, int64_t ldc, double alpha, int beta_zero)
+{
+  const int64_t st = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < mn; i += st) {
+    double s = 0.0;
+    for (int z = 0; z < nz; ++z) s += (double)part[(int64_t)z * mn + i];
+    const int64_t row = i % m, col = i / m, off = row + col * ldc;
+    double v = alpha * s;
+    if (!beta_zero) v += (double)c[off];
+    c[off] = (float)v;
+  }
+}
+
 template <typename K>
 int launch_grid(K kern, const GemmParams &p, hipStream_t s)
 {
@@ -496,7 +631,7 @@ int64_t round_up(int64_t x, int64_t q) { return (x + q - 1) / q * q; }
 int launch_bf16x3(const GemmArgs &g, void *ws, hipStream_t s)
 {
   const int64_t kp = round_up(std::max<int64_t>((int64_t)g.k, 1), 32);
-  const int64_t mp = round_up((int64_t)g.m, kB3Tile), np = round_up((int64_t)g.n, kB3Tile);
+  const int64_t mp = round_up((int64_t)g.m, kB3M), np = round_up((int64_t)g.n, kB3N);
   if (!ws) return TALSH_INVALID_ARGS;
   unsigned short *a3 = static_cast<unsigned short *>(ws);
   unsigned short *b3 = a3 + 3 * mp * kp;
@@ -518,7 +653,22 @@ int launch_bf16x3(const GemmArgs &g, void *ws, hipStream_t s)
   p.m = (int64_t)g.m; p.n = (int64_t)g.n; p.k_pad = kp; p.a_rows_pad = mp; p.b_rows_pad = np; p.ldc = (int64_t)g.m;
   p.alpha = g.alpha_re; p.beta_zero = g.beta_zero;
   p.tiles_m = (int)(mp / kB3Tile); p.tiles_n = (int)(np / kB3Tile);
-  hipLaunchKernelGGL((k_gemm_bf16x3<kB3Tile, kB3Tile>), dim3(p.tiles_m * p.tiles_n), dim3(256), 0, s, p);
+  if (g.tile == 1) {  // single-pass form: FP64 folds in registers, one wave per SIMD
+    hipLaunchKernelGGL((k_gemm_bf16x3<kB3Tile, kB3Tile>), dim3(p.tiles_m * p.tiles_n), dim3(256), 0, s, p);
+    return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
+  }
+  B3Split q{};
+  q.p = p;
+  q.p.tiles_m = (int)(mp / kB3M);
+  q.p.tiles_n = (int)(np / kB3N);
+  q.kchunk = std::min<int64_t>(kp, kB3Chunk);
+  const int nz = (int)((kp + q.kchunk - 1) / q.kchunk);
+  q.part = reinterpret_cast<float *>(b3 + 3 * np * kp);
+  hipLaunchKernelGGL(k_gemm_bf16x3v3, dim3(q.p.tiles_m * q.p.tiles_n, nz), dim3(512), 0, s, q);
+  const int64_t mn = (int64_t)g.m * (int64_t)g.n;
+  const int rb = (int)std::min<int64_t>((mn + 255) / 256, 16384);
+  hipLaunchKernelGGL(k_splitk_reduce, dim3(rb), dim3(256), 0, s, q.part, nz, mn, (int64_t)g.m, p.c, p.ldc, p.alpha,
+                     p.beta_zero);
   return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
 }
 
@@ -528,11 +678,33 @@ size_t gemm_workspace_bytes(const GemmArgs &g)
 {
   if (g.kind != R4 || g.r4mode != R4Mode::kBF16x3) return 0;
   const int64_t kp = round_up(std::max<int64_t>((int64_t)g.k, 1), 32);
-  const int64_t mp = round_up((int64_t)g.m, kB3Tile), np = round_up((int64_t)g.n, kB3Tile);
-  return (size_t)(3 * (mp + np) * kp) * sizeof(unsigned short);
+  const int64_t mp = round_up((int64_t)g.m, kB3M), np = round_up((int64_t)g.n, kB3N);
+  const int64_t nz = (kp + kB3Chunk - 1) / kB3Chunk;
+  // split planes (bf16) + K-chunk partials (FP32, m x n each; 16-byte aligned start)
+  return (size_t)(3 * (mp + np) * kp) * sizeof(unsigned short) + (size_t)(nz * (int64_t)g.m * (int64_t)g.n) * 4 + 16;
 }
 
 int launch_gemm(const GemmArgs &g, hipStream_t s) { return launch_gemm_ws(g, nullptr, s); }
+
+int gemm_tile_choices(int kind, R4Mode r4mode);
+
+int gemm_default_tile(int kind, R4Mode r4mode, size_t m, size_t n, size_t k)
+{
+  if (gemm_tile_choices(kind, r4mode) == 1) return 0;
+  static const int cus = []() {
+    int dev = 0, c = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+        prop.multiProcessorCount > 0)
+      c = prop.multiProcessorCount;
+    return c;
+  }();
+  // 128x128 tiles pay off only when there are enough of them to fill the
+  // chip twice over and K is deep enough to amortise their prologue;
+  // otherwise 64x64 tiles balance better (measured over random contractions)
+  const size_t tiles128 = ((m + 127) / 128) * ((n + 127) / 128);
+  return (tiles128 >= (size_t)(2 * cus) && k >= 2048) ? 0 : 2;
+}
 
 int gemm_tile_choices(int kind, R4Mode r4mode)
 {
@@ -570,7 +742,7 @@ int launch_gemm_ws(const GemmArgs &g, void *ws, hipStream_t s)
     p.vec_b = (((uintptr_t)g.b % 16) == 0) && ((p.ldb * es) % 16 == 0);
   }
   if ((p.m + 63) / 64 * ((p.n + 63) / 64) >= (1ll << 31)) return TALSH_LIMIT_EXCEEDED;
-  const int tile = g.tile < 0 ? 0 : g.tile;
+  const int tile = g.tile < 0 ? gemm_default_tile(g.kind, g.r4mode, g.m, g.n, g.k) : g.tile;
   switch (g.kind) {
   case R8:
     switch (tile) {

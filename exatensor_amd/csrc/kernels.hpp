@@ -59,5 +59,7 @@ int launch_gemm_ws(const GemmArgs &g, void *ws, hipStream_t s);
 // (bm x bn) and occupancy target of a choice.
 int gemm_tile_choices(int kind, R4Mode r4mode);
 void gemm_tile_shape(int kind, R4Mode r4mode, int tile, int *bm, int *bn, int *occupancy);
+// Tile choice of launch_gemm when GemmArgs::tile < 0.
+int gemm_default_tile(int kind, R4Mode r4mode, size_t m, size_t n, size_t k);
 
 }  // namespace xt

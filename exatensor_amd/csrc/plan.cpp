@@ -287,6 +287,17 @@ bool selector_choose(const std::vector<int> &dig, int dr, int lr, int rr, const 
 {
   std::lock_guard<std::mutex> g(g_sel_mu);
   if (!g_sel) return false;
+  {  // outside the model's training domain the heuristic plans
+    double fl = 2.0, vl = 1, vr = 1, vd = 1;  // real flop count, as the domain is defined
+    for (size_t i = 0; i < ld.size(); ++i) { fl *= ld[i]; vl *= ld[i]; }
+    for (size_t j = 0; j < rd.size(); ++j) {
+      vr *= rd[j];
+      if (dig[lr + j] > 0) fl *= rd[j];
+    }
+    for (int x : dd) vd *= x;
+    if (std::log2(fl) > kSelectorMaxLog2Flops || std::log2(std::max(vl, std::max(vr, vd))) > kSelectorMaxLog2Volume)
+      return false;
+  }
   std::string key;
   key.reserve(64);
   auto put = [&](int v) { key.append(std::to_string(v)).push_back(','); };

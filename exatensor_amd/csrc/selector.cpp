@@ -384,8 +384,13 @@ int talshAmdSelectorChoice(const char *cptrn, int drank, const int *ddims, int l
   const int cus = (E.on && !E.gpus.empty()) ? E.gpus[0]->num_cus : 256;
   const R4Mode mode = fast_math ? R4Mode::kBF16x3 : R4Mode::kNativeF32;
   if (!selector_choose(dig, dr, lr, rr, dd, ld, rd, data_kind, mode, accumulative != NOPE, cus, variant, tile)) {
-    *variant = 0;  // the byte-count heuristic's choice
-    *tile = -1;
+    // the heuristic: fewest transpose bytes, default tile for the GEMM shape
+    ContractPlan P;
+    const int e = plan_contraction(dig, dr, lr, rr, dd, ld, rd, kind_size(data_kind), accumulative != NOPE, -1, P,
+                                   nullptr);
+    if (e) return e;
+    *variant = 0;
+    *tile = gemm_default_tile(data_kind, mode, P.swap ? P.n : P.m, P.swap ? P.m : P.n, P.k);
   }
   return 0;
 }

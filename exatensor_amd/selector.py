@@ -244,28 +244,35 @@ class Mlp:
             self.h = C.c_void_p()
 
 
-def regret_report(lib, specs, times, model=None):
-    """Regret of the installed selector's choice (model installed first when
-    given) and of the byte-count heuristic, against exhaustive search."""
-    if model is not None:
-        model.install()
+def choice(lib, spec):
+    """(variant, tile) that talshTensorContract would run for this contraction."""
+    v, t = C.c_int(), C.c_int()
+    e = lib.talshAmdSelectorChoice(spec["pattern"].encode(), len(spec["ddims"]), T.ints(spec["ddims"]),
+                                   len(spec["ldims"]), T.ints(spec["ldims"]), len(spec["rdims"]), T.ints(spec["rdims"]),
+                                   spec["kind"], T.YEP if spec["acc"] else T.NOPE, 0, C.byref(v), C.byref(t))
+    if e:
+        raise RuntimeError(f"talshAmdSelectorChoice: {e}")
+    return v.value, max(t.value, 0)
+
+
+def regret_report(lib, specs, times, model):
+    """Regret of the model's choices and of the byte-count heuristic's
+    (variant 0 with its default tile), against exhaustive search. Leaves the
+    model installed."""
+    lib.talshAmdSelectorInstall(None)
+    heur_pick = [choice(lib, s) for s in specs]
+    model.install()
     sel, heur, tsel, theur, tbest = [], [], 0.0, 0.0, 0.0
-    for spec, tm in zip(specs, times):
+    for spec, tm, hp in zip(specs, times, heur_pick):
         variants, tiles, _ = plan_features(lib, spec)
-        v, t = C.c_int(), C.c_int()
-        e = lib.talshAmdSelectorChoice(spec["pattern"].encode(), len(spec["ddims"]), T.ints(spec["ddims"]),
-                                       len(spec["ldims"]), T.ints(spec["ldims"]), len(spec["rdims"]),
-                                       T.ints(spec["rdims"]), spec["kind"], T.YEP if spec["acc"] else T.NOPE, 0,
-                                       C.byref(v), C.byref(t))
-        if e:
-            raise RuntimeError(f"talshAmdSelectorChoice: {e}")
-        pick = [i for i in range(len(variants)) if variants[i] == v.value and tiles[i] == max(t.value, 0)]
-        h = [i for i in range(len(variants)) if variants[i] == 0 and tiles[i] == 0]
+        mp = choice(lib, spec)
+        index = {(int(a), int(b)): i for i, (a, b) in enumerate(zip(variants, tiles))}
         best = float(np.min(tm))
-        sel.append(tm[pick[0]] / best - 1.0)
-        heur.append(tm[h[0]] / best - 1.0)
-        tsel += tm[pick[0]]
-        theur += tm[h[0]]
+        ts, th = tm[index[mp]], tm[index[hp]]
+        sel.append(ts / best - 1.0)
+        heur.append(th / best - 1.0)
+        tsel += ts
+        theur += th
         tbest += best
     sel, heur = np.array(sel), np.array(heur)
     return dict(contractions=len(specs), mean_regret=float(sel.mean()), median_regret=float(np.median(sel)),

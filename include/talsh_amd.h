@@ -99,12 +99,15 @@ int talshAmdPlanFeatures(const char *cptrn, int drank, const int *ddims, int lra
                          const int *rdims, int data_kind, int accumulative, int fast_math, int cap, int *variants,
                          int *tiles, float *features, int *ncand);
 /* Install a copy of a trained model as the planner of talshTensorContract
- * (NULL removes it; without a model the byte-count heuristic plans). The
- * environment variable TALSH_PLAN_MODEL names a model file loaded at
- * talshInit. */
+ * (NULL removes it; without a model the byte-count heuristic plans, and so it
+ * does for contractions beyond the model's training domain of 2^35.5 flops or
+ * 2^26.5-element operands). talshInit loads the model file named by the
+ * environment variable TALSH_PLAN_MODEL, or else plan_model.txt next to
+ * libtalsh.so when present; TALSH_PLAN_MODEL set to an empty string disables
+ * the selector. */
 int talshAmdSelectorInstall(void *mlp);
-/* The candidate the installed selector picks (variant 0, tile -1: the
- * heuristic's choice when no model is installed). */
+/* The candidate talshTensorContract would run: the installed selector's
+ * choice, or the heuristic's (variant 0 with its default GEMM tile). */
 int talshAmdSelectorChoice(const char *cptrn, int drank, const int *ddims, int lrank, const int *ldims, int rrank,
                            const int *rdims, int data_kind, int accumulative, int fast_math, int *variant, int *tile);
 

@@ -1,0 +1,60 @@
+"""Matrix kernels on awkward shapes through talshAmdGemm: the three-way BF16
+split (both kernel forms: split-K two-level and single-pass) against FP64,
+ragged m/n/k, every operand orientation, beta 0/1 and K chunk edges."""
+import numpy as np
+import pytest
+
+from exatensor_amd import talsh as T
+from gpu_util import engine, gpu_tensor, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _gemm(m, n, k, ak, bk, beta_zero, tile, fast, kind=T.R4, seed=0):
+    eng = engine()
+    L = eng.lib
+    rng = np.random.default_rng(seed)
+    dt = np.float32 if kind == T.R4 else np.float64
+    a = rng.uniform(-1, 1, m * k).astype(dt)
+    b = rng.uniform(-1, 1, n * k).astype(dt)
+    c0 = rng.uniform(-1, 1, m * n).astype(dt)
+    A = a.reshape(m, k) if ak else a.reshape(k, m).T
+    B = b.reshape(n, k) if bk else b.reshape(k, n).T
+    alpha = 0.75
+    want = alpha * (A.astype(np.float64) @ B.astype(np.float64).T)
+    if not beta_zero:
+        want += c0.reshape(n, m).T.astype(np.float64)
+    ta, tb, tc = gpu_tensor(a, [a.size]), gpu_tensor(b, [b.size]), gpu_tensor(c0, [c0.size])
+    wsb = L.talshAmdGemmWorkspace(kind, fast, m, n, k)
+    tw = T.Tensor(L, T.R4, [max(1, (wsb + 3) // 4)], eng.gpu(0), 0.0) if wsb else None
+    ptr = lambda t: t.body(eng.gpu(0))[1] if t is not None else None
+    e = L.talshAmdGemm(kind, fast, m, n, k, ptr(ta), ak, ptr(tb), bk, ptr(tc), alpha, 0.0, 1 if beta_zero else 0,
+                       tile, ptr(tw), wsb)
+    got = tc.download().reshape(n, m).T if e == 0 else None
+    for t in (ta, tb, tc, tw):
+        if t is not None:
+            t.destruct()
+    return e, got, want
+
+
+@pytest.mark.parametrize("tile", [-1, 1])
+@pytest.mark.parametrize("m,n,k,ak,bk,beta_zero", [
+    (1, 1, 1, 1, 1, True),
+    (300, 200, 70, 1, 1, True),
+    (257, 129, 2100, 1, 1, False),
+    (129, 300, 33, 0, 1, True),
+    (200, 130, 4500, 1, 0, True),
+    (513, 255, 2048, 0, 0, False),
+    (1000, 700, 6200, 1, 1, True),
+])
+def test_bf16x3_shapes_against_fp64(m, n, k, ak, bk, beta_zero, tile):
+    e, got, want = _gemm(m, n, k, ak, bk, beta_zero, tile, fast=1)
+    assert e == 0
+    assert rel_err(got, want) <= 1e-6, (m, n, k, tile)
+
+
+@pytest.mark.parametrize("kind,tile", [(T.R4, 0), (T.R4, 2), (T.R8, 0), (T.R8, 1), (T.R8, 2)])
+def test_native_kernels_ragged(kind, tile):
+    e, got, want = _gemm(333, 177, 1001, 1, 0, False, tile, fast=0, kind=kind, seed=3)
+    assert e == 0
+    assert rel_err(got, want) <= (1e-5 if kind == T.R4 else 1e-13)
