@@ -696,21 +696,75 @@ int talshTensorScale_(talsh_tens_t *dtens, double vr, double vi, int dev_id, int
 }
 
 // ============================================================================ slice / insert (not on this engine)
-int talshTensorSlice(talsh_tens_t *, talsh_tens_t *, const int *, int, int, int, int, talsh_task_t *)
+}  // extern "C"
+
+// Slice (D = L[offsets + x], D the box) or insert (D[offsets + x] = L, L the
+// box), accumulating when asked: TALSH/talshc.cpp:4008-4300 (talshTensorSlice,
+// talshTensorInsert) over TALSH/tensor_algebra_cpu.F90:2315-2620; the
+// reference runs them on the host only (its GPU entries are stubs,
+// TALSH/tensor_algebra_gpu_nvidia.cu:4788-4804).
+static int box_op(talsh_tens_t *dtens, talsh_tens_t *ltens, const int *offsets, int dev_id, int dev_kind,
+                  int copy_ctrl, int accumulative, talsh_task_t *task, bool insert)
 {
-  return TALSH_NOT_IMPLEMENTED;
+  int err = 0;
+  Device *d = exec_device(dev_id, dev_kind, &err);
+  if (!d) return err;
+  if (!dtens || !ltens) return TALSH_INVALID_ARGS;
+  if (talshTensorIsEmpty(dtens) == YEP || talshTensorIsEmpty(ltens) == YEP) return TALSH_OBJECT_IS_EMPTY;
+  const int rank = talshTensorRank(dtens);
+  if (rank != talshTensorRank(ltens) || (rank > 0 && !offsets)) return TALSH_INVALID_ARGS;
+  const std::vector<int> dd = dims_of(dtens), ld = dims_of(ltens);
+  const std::vector<int> &big = insert ? dd : ld, &box = insert ? ld : dd;
+  std::vector<int64_t> bigstr(rank), boxstr(rank);
+  int64_t base = 0, sb = 1, sx = 1;
+  for (int i = 0; i < rank; ++i) {
+    if (offsets[i] < 0 || box[i] <= 0 || (int64_t)offsets[i] + box[i] > big[i]) return TALSH_INVALID_ARGS;
+    bigstr[i] = sb;
+    boxstr[i] = sx;
+    base += (int64_t)offsets[i] * sb;
+    sb *= big[i];
+    sx *= box[i];
+  }
+  const bool acc = accumulative != NOPE;
+  std::vector<Arg> args(2);
+  args[0].t = dtens; args[0].output = true; args[0].need_input = acc || insert;  // insert keeps D outside the box
+  args[1].t = ltens;
+  struct Body {
+    Arg *D, *L; std::vector<int> box; std::vector<int64_t> sstr, dstr; int64_t sbase, dbase; bool acc;
+    int allocate(Temps &) { return 0; }
+    int enqueue(Task *k, Device *)
+    {
+      return launch_box(L->exec, D->exec, L->kind, (int)box.size(), box.data(), sstr.data(), dstr.data(), sbase,
+                        dbase, acc, k->stream);
+    }
+  } body{&args[0], &args[1], box, insert ? boxstr : bigstr, insert ? bigstr : boxstr, insert ? 0 : base,
+         insert ? base : 0, acc};
+  double vol = 1;
+  for (int x : box) vol *= x;
+  return run_task(task, d, (unsigned)copy_ctrl, args, tensor_kind(dtens), acc ? vol : 0.0, body);
 }
-int talshTensorSlice_(talsh_tens_t *, talsh_tens_t *, const int *, int, int, int, int, talsh_task_t *)
+
+extern "C" {
+
+int talshTensorSlice(talsh_tens_t *dtens, talsh_tens_t *ltens, const int *offsets, int dev_id, int dev_kind,
+                     int copy_ctrl, int accumulative, talsh_task_t *task)
 {
-  return TALSH_NOT_IMPLEMENTED;
+  return box_op(dtens, ltens, offsets, dev_id, dev_kind, copy_ctrl, accumulative, task, false);
 }
-int talshTensorInsert(talsh_tens_t *, talsh_tens_t *, const int *, int, int, int, int, talsh_task_t *)
+int talshTensorSlice_(talsh_tens_t *dtens, talsh_tens_t *ltens, const int *offsets, int dev_id, int dev_kind,
+                      int copy_ctrl, int accumulative, talsh_task_t *task)
 {
-  return TALSH_NOT_IMPLEMENTED;
+  return talshTensorSlice(dtens, ltens, offsets, dev_id, dev_kind, copy_ctrl, accumulative, task);
 }
-int talshTensorInsert_(talsh_tens_t *, talsh_tens_t *, const int *, int, int, int, int, talsh_task_t *)
+int talshTensorInsert(talsh_tens_t *dtens, talsh_tens_t *ltens, const int *offsets, int dev_id, int dev_kind,
+                      int copy_ctrl, int accumulative, talsh_task_t *task)
 {
-  return TALSH_NOT_IMPLEMENTED;
+  return box_op(dtens, ltens, offsets, dev_id, dev_kind, copy_ctrl, accumulative, task, true);
+}
+int talshTensorInsert_(talsh_tens_t *dtens, talsh_tens_t *ltens, const int *offsets, int dev_id, int dev_kind,
+                       int copy_ctrl, int accumulative, talsh_task_t *task)
+{
+  return talshTensorInsert(dtens, ltens, offsets, dev_id, dev_kind, copy_ctrl, accumulative, task);
 }
 
 // ============================================================================ copy / add

@@ -6,7 +6,9 @@
 // gpu_array_dot_product__ (TALSH/tensor_algebra_gpu_nvidia.cu:337-683).
 // Arithmetic uses explicitly rounded products and sums (no fused multiply-add)
 // so results equal the CPU path's separate multiply and add bit for bit.
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "kernels.hpp"
 
@@ -202,7 +204,101 @@ double real_factor(double re, double im)
   return im;
 }
 
+// ---- box copy: tensor slice and insert ----------------------------------------
+// out[dbase + sum_i x_i ds_i] (+)= in[sbase + sum_i x_i ss_i] over the box
+// x < ext (dimension 0 fastest, its strides 1 on both sides after fusion).
+constexpr int kBoxDims = 8;
+struct BoxDiv { uint32_t d, m, s; };
+struct BoxParams {
+  int nd;                       // box dimensions after the first (row decode)
+  BoxDiv e0;                    // extent of dimension 0
+  BoxDiv ext[kBoxDims];         // extents of dimensions 1..nd
+  int64_t ss[kBoxDims], ds[kBoxDims];
+  int64_t sbase, dbase;
+  uint64_t vol;
+};
+
+__device__ __forceinline__ uint32_t bdiv(uint32_t n, const BoxDiv &f)
+{
+  return f.d == 1 ? n : ((__umulhi(n, f.m) + n) >> f.s);
+}
+
+template <typename R, int NC>
+__global__ __launch_bounds__(kNT) void k_box(const R *__restrict__ in, R *__restrict__ out, const BoxParams p, int acc)
+{
+  const uint64_t st = (uint64_t)gridDim.x * kNT;
+  for (uint64_t g = (uint64_t)blockIdx.x * kNT + threadIdx.x; g < p.vol; g += st) {
+    // g < 2^31 for 32-bit decode (checked by the launcher)
+    uint32_t r = bdiv((uint32_t)g, p.e0);
+    const uint32_t x0 = (uint32_t)g - r * p.e0.d;
+    int64_t so = p.sbase + x0, dof = p.dbase + x0;
+    for (int q = 0; q < p.nd; ++q) {
+      const uint32_t rq = bdiv(r, p.ext[q]);
+      const uint32_t c = r - rq * p.ext[q].d;
+      r = rq;
+      so += (int64_t)c * p.ss[q];
+      dof += (int64_t)c * p.ds[q];
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const R v = in[so * NC + c];
+      out[dof * NC + c] = acc ? add(out[dof * NC + c], v) : v;
+    }
+  }
+}
+
 }  // namespace
+
+int launch_box(const void *in, void *out, int kind, int rank, const int *box, const int64_t *sstr,
+               const int64_t *dstr, int64_t sbase, int64_t dbase, bool acc, hipStream_t s)
+{
+  if (!kind_size(kind) || rank < 0) return TALSH_INVALID_ARGS;
+  // drop unit extents; fuse dimension i+1 into i when both sides are contiguous across them
+  std::vector<int64_t> e, a, b;
+  for (int i = 0; i < rank; ++i) {
+    if (box[i] <= 0) return TALSH_INVALID_ARGS;
+    if (box[i] == 1) continue;
+    if (!e.empty() && a.back() * e.back() == sstr[i] && b.back() * e.back() == dstr[i]) {
+      e.back() *= box[i];
+    } else {
+      e.push_back(box[i]);
+      a.push_back(sstr[i]);
+      b.push_back(dstr[i]);
+    }
+  }
+  uint64_t vol = 1;
+  for (int64_t x : e) vol *= (uint64_t)x;
+  if (e.empty()) { e.push_back(1); a.push_back(1); b.push_back(1); }
+  if (a[0] != 1 || b[0] != 1) {  // innermost box dimension not contiguous: keep it as a row dimension
+    e.insert(e.begin(), 1);
+    a.insert(a.begin(), 1);
+    b.insert(b.begin(), 1);
+  }
+  if ((int)e.size() - 1 > kBoxDims || vol >= (1ull << 31) || e[0] >= (1ll << 31)) return TALSH_LIMIT_EXCEEDED;
+  BoxParams p{};
+  auto mk = [](int64_t d) { FastDiv f((uint32_t)d); return BoxDiv{f.d, f.m, f.s}; };
+  p.e0 = mk(e[0]);
+  p.nd = (int)e.size() - 1;
+  for (int q = 0; q < p.nd; ++q) {
+    p.ext[q] = mk(e[q + 1]);
+    p.ss[q] = a[q + 1];
+    p.ds[q] = b[q + 1];
+  }
+  p.sbase = sbase;
+  p.dbase = dbase;
+  p.vol = vol;
+  if (vol == 0) return 0;
+  const int blocks = (int)std::min<uint64_t>((vol + kNT - 1) / kNT, 65536);
+  const int ac = acc ? 1 : 0;
+  switch (kind) {
+  case R4: hipLaunchKernelGGL((k_box<float, 1>), dim3(blocks), dim3(kNT), 0, s, (const float *)in, (float *)out, p, ac); break;
+  case R8: hipLaunchKernelGGL((k_box<double, 1>), dim3(blocks), dim3(kNT), 0, s, (const double *)in, (double *)out, p, ac); break;
+  case C4: hipLaunchKernelGGL((k_box<float, 2>), dim3(blocks), dim3(kNT), 0, s, (const float *)in, (float *)out, p, ac); break;
+  case C8: hipLaunchKernelGGL((k_box<double, 2>), dim3(blocks), dim3(kNT), 0, s, (const double *)in, (double *)out, p, ac); break;
+  default: return TALSH_INVALID_ARGS;
+  }
+  return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
+}
 
 int launch_fill(void *d, int kind, size_t vol, double re, double im, hipStream_t s)
 {

@@ -556,15 +556,20 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3v3(const B3Split q)
       us8 fa[3];
 #pragma unroll
       for (int pl = 0; pl < 3; ++pl) fa[pl] = *reinterpret_cast<const us8 *>(&As[cur][pl][wm + i * 16 + fr][fpos]);
+      // products outer, blocks inner: consecutive MFMAs write different
+      // accumulators (smallest terms first, as in the single-pass form)
 #pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        acc[i][j] = mfma_bf(fa[0], fb[2][j], acc[i][j]);
-        acc[i][j] = mfma_bf(fa[1], fb[1][j], acc[i][j]);
-        acc[i][j] = mfma_bf(fa[2], fb[0][j], acc[i][j]);
-        acc[i][j] = mfma_bf(fa[0], fb[1][j], acc[i][j]);
-        acc[i][j] = mfma_bf(fa[1], fb[0][j], acc[i][j]);
-        acc[i][j] = mfma_bf(fa[0], fb[0][j], acc[i][j]);
-      }
+      for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[0], fb[2][j], acc[i][j]);
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[1], fb[1][j], acc[i][j]);
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[2], fb[0][j], acc[i][j]);
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[0], fb[1][j], acc[i][j]);
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[1], fb[0][j], acc[i][j]);
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[0], fb[0][j], acc[i][j]);
     }
     if ((kt + 1) % kFold == 0 || kt + 1 == nk) {
 #pragma unroll

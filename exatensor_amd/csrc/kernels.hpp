@@ -35,6 +35,11 @@ int launch_axpy(void *d, const void *l, int kind, size_t vol, double re, double 
 int launch_dot(void *d, const void *l, const void *r, int kind, size_t n, double are, double aim, bool lconj,
                bool rconj, bool accumulate, void *scratch, size_t scratch_bytes, hipStream_t s);
 
+// out[dbase + sum_i x_i dstr_i] (+)= in[sbase + sum_i x_i sstr_i] for every x
+// in the box 0 <= x_i < box[i] (tensor slice / insert; element strides).
+int launch_box(const void *in, void *out, int kind, int rank, const int *box, const int64_t *sstr,
+               const int64_t *dstr, int64_t sbase, int64_t dbase, bool acc, hipStream_t s);
+
 // ---- matrix products (k_gemm.hip) -----------------------------------------
 // C(m x n, column-major, ldc = m) = beta*C + alpha * op(A) * op(B)
 //   a_kfast: A stored k x m with k fastest (else m x k with m fastest)

@@ -540,6 +540,75 @@ int cptal_tensor_copy(int kind, int rank, const int *ldims, const int *dig, cons
   return cptal_copy_dlf(kind, rank, ldims, trn, l, d, lconj);
 }
 
+/* Element update of slice/insert: out = out * beta + in (alpha = 1), in the
+ * kind's own arithmetic. Complex factors are (beta, 0) and (1, 0), so the
+ * complex products reduce to the componentwise forms below. */
+static void box_update(int kind, void *out, size_t o, const void *in, size_t i, int beta, int full)
+{
+  switch (kind) {
+  case CPTAL_R4: {
+    float *y = (float *)out; const float *x = (const float *)in;
+    y[o] = full ? (beta ? y[o] + x[i] : x[i]) : y[o] * (float)beta + x[i] * 1.0f;
+    break;
+  }
+  case CPTAL_R8: {
+    double *y = (double *)out; const double *x = (const double *)in;
+    y[o] = full ? (beta ? y[o] + x[i] : x[i]) : y[o] * (double)beta + x[i] * 1.0;
+    break;
+  }
+  case CPTAL_C4: {
+    float *y = (float *)out; const float *x = (const float *)in;
+    for (int c = 0; c < 2; ++c)
+      y[2 * o + c] = full ? (beta ? y[2 * o + c] + x[2 * i + c] : x[2 * i + c]) : y[2 * o + c] * (float)beta + x[2 * i + c];
+    break;
+  }
+  case CPTAL_C8: {
+    double *y = (double *)out; const double *x = (const double *)in;
+    for (int c = 0; c < 2; ++c)
+      y[2 * o + c] = full ? (beta ? y[2 * o + c] + x[2 * i + c] : x[2 * i + c]) : y[2 * o + c] * (double)beta + x[2 * i + c];
+    break;
+  }
+  }
+}
+
+/* Walks the slice box in DLF order; insert = 0: slice <- tens, 1: tens <- slice. */
+static int box_walk(int kind, int rank, const int *tdims, void *tens, const int *sdims, void *slice, const int *offsets,
+                    int accumulative, int insert)
+{
+  if (!cptal_kind_size(kind) || rank <= 0 || rank > CPTAL_MAX_RANK) return 1;
+  int full = 1;
+  for (int i = 0; i < rank; ++i) {
+    if (offsets[i] < 0 || offsets[i] >= tdims[i] || sdims[i] <= 0 || offsets[i] + sdims[i] - 1 >= tdims[i]) return 17;
+    if (sdims[i] != tdims[i]) full = 0;
+  }
+  size_t tbase[CPTAL_MAX_RANK], svol = 1, st = 1;
+  for (int i = 0; i < rank; ++i) { tbase[i] = st; st *= (size_t)tdims[i]; svol *= (size_t)sdims[i]; }
+  int idx[CPTAL_MAX_RANK] = {0};
+  for (size_t j = 0; j < svol; ++j) {
+    size_t t = 0;
+    for (int i = 0; i < rank; ++i) t += (size_t)(offsets[i] + idx[i]) * tbase[i];
+    if (insert) box_update(kind, tens, t, slice, j, accumulative != 0, full);
+    else box_update(kind, slice, j, tens, t, accumulative != 0, full);
+    for (int i = 0; i < rank; ++i) {
+      if (++idx[i] < sdims[i]) break;
+      idx[i] = 0;
+    }
+  }
+  return 0;
+}
+
+int cptal_slice(int kind, int rank, const int *tdims, const void *tens, const int *sdims, void *slice,
+                const int *offsets, int accumulative)
+{
+  return box_walk(kind, rank, tdims, (void *)tens, sdims, slice, offsets, accumulative, 0);
+}
+
+int cptal_insert(int kind, int rank, const int *tdims, void *tens, const int *sdims, const void *slice,
+                 const int *offsets, int accumulative)
+{
+  return box_walk(kind, rank, tdims, tens, sdims, (void *)slice, offsets, accumulative, 1);
+}
+
 int cptal_init(int kind, size_t vol, void *d, double re, double im)
 {
   switch (kind) {

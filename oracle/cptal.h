@@ -64,6 +64,21 @@ int cptal_copy_dlf(int kind, int rank, const int *dims_in, const int *o2n, const
 int cptal_tensor_copy(int kind, int rank, const int *ldims, const int *dig, const void *l, void *d,
                       int conj_bits);
 
+/* Slice extraction, TALSH/tensor_algebra_cpu.F90:2315-2465 (tensor_block_slice)
+ * and :6900-6974 (tensor_block_slice_dlf_*): with beta = accumulative ? 1 : 0,
+ *   slice[j] = slice[j] * beta + tens[offsets + j]   (a slice of the full
+ * extent on every dimension is a copy or an add, as the reference does).
+ * tdims: extents of tens; sdims: extents of slice; offsets 0-based.
+ * Returns 0, or 17 (the reference's code) when the slice exceeds tens. */
+int cptal_slice(int kind, int rank, const int *tdims, const void *tens, const int *sdims, void *slice,
+                const int *offsets, int accumulative);
+
+/* Slice insertion, TALSH/tensor_algebra_cpu.F90:2466-2620 (tensor_block_insert)
+ * and :7216-7290 (tensor_block_insert_dlf_*):
+ *   tens[offsets + j] = tens[offsets + j] * beta + slice[j]. */
+int cptal_insert(int kind, int rank, const int *tdims, void *tens, const int *sdims, const void *slice,
+                 const int *offsets, int accumulative);
+
 /* Fill, TALSH/tensor_algebra_cpu.F90:955 (tensor_block_init). */
 int cptal_init(int kind, size_t vol, void *d, double re, double im);
 

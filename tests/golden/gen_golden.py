@@ -240,5 +240,47 @@ def main():
     print(f"patterns {len(pattern_rows)} permutations {len(perm_rows)} ops {len(meta)}")
 
 
+def slice_insert_fixtures(ref):
+    """talshTensorSlice / talshTensorInsert on the host: random boxes of every
+    data kind, ranks 1..5, accumulative and not, full-extent boxes included."""
+    nrs = np.random.default_rng(11)
+    arrays, meta = {}, []
+
+    def rnd(kind, n):
+        if kind in (C4, C8):
+            return (nrs.uniform(-1, 1, n) + 1j * nrs.uniform(-1, 1, n)).astype(NP_DTYPE[kind])
+        return nrs.uniform(-1, 1, n).astype(NP_DTYPE[kind])
+
+    for i in range(40):
+        kind = (R4, R8, C4, C8)[i % 4]
+        rank = 1 + (i % 5)
+        tdims = [int(nrs.integers(1, 7)) for _ in range(rank)]
+        full = i % 10 == 9
+        sdims = list(tdims) if full else [int(nrs.integers(1, t + 1)) for t in tdims]
+        offs = [0] * rank if full else [int(nrs.integers(0, t - s + 1)) for t, s in zip(tdims, sdims)]
+        acc = bool(i % 3 == 1)
+        tens = rnd(kind, int(np.prod(tdims)))
+        slc = rnd(kind, int(np.prod(sdims)))
+        e1, o1 = ref.slice(tens, tdims, slc, sdims, offs, acc)
+        e2, o2 = ref.insert(tens, tdims, slc, sdims, offs, acc)
+        key = f"box{i}"
+        arrays[key + "_t"] = tens; arrays[key + "_s"] = slc
+        if o1 is not None:
+            arrays[key + "_slice"] = o1
+        if o2 is not None:
+            arrays[key + "_insert"] = o2
+        meta.append(dict(key=key, kind=kind, tdims=tdims, sdims=sdims, offsets=offs, accumulative=acc,
+                         ierr=[e1, e2]))
+    np.savez_compressed(os.path.join(HERE, "slice_insert.npz"), **arrays)
+    with open(os.path.join(HERE, "slice_insert.json"), "w") as f:
+        json.dump(dict(source="reference TAL-SH CPU library, talshTensorSlice/Insert on DEV_HOST", cases=meta), f,
+                  indent=0)
+    print(f"slice/insert cases {len(meta)}")
+
+
 if __name__ == "__main__":
-    main()
+    if "--slice-insert" in sys.argv:
+        slice_insert_fixtures(Reference())
+    else:
+        main()
+        slice_insert_fixtures(Reference())

@@ -41,6 +41,8 @@ class Oracle:
         L.cptal_dig2str.argtypes = [C.c_int, C.c_int, C.c_int, _IP, C.c_char_p, C.c_int]
         L.cptal_copy_dlf.argtypes = [C.c_int, C.c_int, _IP, _IP, C.c_void_p, C.c_void_p, C.c_int]
         L.cptal_tensor_copy.argtypes = [C.c_int, C.c_int, _IP, _IP, C.c_void_p, C.c_void_p, C.c_int]
+        L.cptal_slice.argtypes = [C.c_int, C.c_int, _IP, C.c_void_p, _IP, C.c_void_p, _IP, C.c_int]
+        L.cptal_insert.argtypes = [C.c_int, C.c_int, _IP, C.c_void_p, _IP, C.c_void_p, _IP, C.c_int]
         L.cptal_tensor_add.argtypes = [C.c_int, C.c_int, _IP, _IP, C.c_void_p, C.c_void_p,
                                        C.c_double, C.c_double, C.c_int]
         L.cptal_init.argtypes = [C.c_int, C.c_size_t, C.c_void_p, C.c_double, C.c_double]
@@ -103,6 +105,18 @@ class Oracle:
         return a
 
     # --- operations ---------------------------------------------------------
+    def slice(self, tens, tdims, slc, sdims, offsets, accumulative):
+        out = np.array(slc, copy=True)
+        e = self.lib.cptal_slice(KIND_OF[tens.dtype], len(tdims), _ia(tdims), tens.ctypes.data, _ia(sdims),
+                                 out.ctypes.data, _ia(offsets), 1 if accumulative else 0)
+        return e, out
+
+    def insert(self, tens, tdims, slc, sdims, offsets, accumulative):
+        out = np.array(tens, copy=True)
+        e = self.lib.cptal_insert(KIND_OF[tens.dtype], len(tdims), _ia(tdims), out.ctypes.data, _ia(sdims),
+                                  slc.ctypes.data, _ia(offsets), 1 if accumulative else 0)
+        return e, out
+
     def copy(self, pattern, ldims, l, conj=None):
         e, dig, dr, lr, rr, cj = self.str2dig(pattern)
         assert e == 0, (pattern, e)
@@ -201,6 +215,9 @@ class Reference:
                                       C.c_void_p]
         L.talshTensorScale.argtypes = [C.POINTER(TalshTens), C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                        C.c_void_p]
+        for fn in (L.talshTensorSlice, L.talshTensorInsert):
+            fn.argtypes = [C.POINTER(TalshTens), C.POINTER(TalshTens), _IP, C.c_int, C.c_int, C.c_int, C.c_int,
+                           C.c_void_p]
         if not Reference._inited:
             sz = C.c_size_t(1 << 28)
             mx = C.c_int()
@@ -274,6 +291,28 @@ class Reference:
         s = complex(scale)
         e = self.lib.talshTensorAdd(pattern.encode(), C.byref(td), C.byref(tl), s.real, s.imag, 0, 0, 6, None)
         out = self._read(pd, d) if e == 0 else None
+        for t in (td, tl):
+            self.lib.talshTensorDestruct(C.byref(t))
+        return e, out
+
+    def slice(self, tens, tdims, slc, sdims, offsets, accumulative):
+        """talshTensorSlice on the host: slice (sdims, initial values slc) from tens."""
+        td, pd = self._tensor(slc, sdims)
+        tl, _ = self._tensor(tens, tdims)
+        e = self.lib.talshTensorSlice(C.byref(td), C.byref(tl), _ia(offsets), 0, 0, 6, 1 if accumulative else 0,
+                                      None)
+        out = self._read(pd, slc) if e == 0 else None
+        for t in (td, tl):
+            self.lib.talshTensorDestruct(C.byref(t))
+        return e, out
+
+    def insert(self, tens, tdims, slc, sdims, offsets, accumulative):
+        """talshTensorInsert on the host: slc inserted into tens at offsets."""
+        td, pd = self._tensor(tens, tdims)
+        tl, _ = self._tensor(slc, sdims)
+        e = self.lib.talshTensorInsert(C.byref(td), C.byref(tl), _ia(offsets), 0, 0, 6, 1 if accumulative else 0,
+                                       None)
+        out = self._read(pd, tens) if e == 0 else None
         for t in (td, tl):
             self.lib.talshTensorDestruct(C.byref(t))
         return e, out
