@@ -1053,12 +1053,191 @@ int talshTensorContract_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *l
   return talshTensorContract(cptrn, dtens, ltens, rtens, sr, si, dev_id, dev_kind, copy_ctrl, accumulative, task);
 }
 
+}  // extern "C"
+
+// ---- out-of-device-memory contraction (talshTensorContractXL) -------------------
+static size_t g_xl_budget = 0;  // device bytes one sub-contraction may use (0: half the arena)
+
+namespace {
+
+void *image_body(const talsh_tens_t *t, int flat, int kind)
+{
+  for (int i = 0; i < t->ndev; ++i)
+    if (t->dev_rsc[i].dev_id == flat && t->data_kind[i] == kind && t->avail[i] == YEP) return t->dev_rsc[i].gmem_p;
+  return nullptr;
+}
+
+// Copies the box [offs, offs + box) of a host DLF body to (gather) or from a
+// contiguous buffer, one run of box[0] elements at a time.
+void host_box(char *full, const int *fdims, const size_t *offs, const int *box, int rank, size_t es, char *buf,
+              bool gather)
+{
+  if (rank == 0) {
+    if (gather) std::memcpy(buf, full, es);
+    else std::memcpy(full, buf, es);
+    return;
+  }
+  size_t rows = 1;
+  for (int i = 1; i < rank; ++i) rows *= (size_t)box[i];
+  std::vector<size_t> fstr(rank);
+  size_t st = 1;
+  for (int i = 0; i < rank; ++i) { fstr[i] = st; st *= (size_t)fdims[i]; }
+  const size_t rowb = (size_t)box[0] * es;
+  std::vector<int> idx(rank, 0);
+  for (size_t r = 0; r < rows; ++r) {
+    size_t off = offs[0];
+    for (int i = 1; i < rank; ++i) off += (offs[i] + (size_t)idx[i]) * fstr[i];
+    char *f = full + off * es, *b = buf + r * rowb;
+    if (gather) std::memcpy(b, f, rowb);
+    else std::memcpy(f, b, rowb);
+    for (int i = 1; i < rank; ++i) {
+      if (++idx[i] < box[i]) break;
+      idx[i] = 0;
+    }
+  }
+}
+
+size_t slice_bytes(const talsh_tens_slice_t &s, size_t es) { return tensShape_volume(&s.shape) * es; }
+
+}  // namespace
+
+extern "C" {
+
+// Contractions too large for the device run as a sequence of sub-contractions
+// on slices: the operation is halved with talshTensorOpDecompose2 until every
+// piece fits the budget, as in TALSH/talshc.cpp:5394-5642. Each piece gathers
+// its L, R and current D slices (on the GPU with talshTensorSlice when the full
+// tensor has a GPU image, else from the host image through pinned staging),
+// accumulates L*R into the D slice on the GPU, and writes the D slice back.
+// Pieces run in order, so pieces that share a D region (a split contracted
+// index) accumulate one after the other.
 int talshTensorContractXL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens, double sr,
                           double si, int dev_id, int dev_kind, int accumulative)
 {
-  // Operands that fit the device arena run as one blocking contraction;
-  // out-of-memory decomposition is not provided by this engine.
-  return talshTensorContract(cptrn, dtens, ltens, rtens, sr, si, dev_id, dev_kind, COPY_MTT, accumulative, nullptr);
+  int err = 0;
+  Device *d = exec_device(dev_id, dev_kind, &err);
+  if (!d) return err;
+  if (!cptrn || !dtens || !ltens || !rtens) return TALSH_INVALID_ARGS;
+  if (talshTensorIsEmpty(dtens) == YEP || talshTensorIsEmpty(ltens) == YEP || talshTensorIsEmpty(rtens) == YEP)
+    return TALSH_OBJECT_IS_EMPTY;
+  const int kind = tensor_kind(dtens);
+  const size_t es = kind_size(kind);
+  const size_t dsz = tensor_volume(dtens) * es, lsz = tensor_volume(ltens) * es, rsz = tensor_volume(rtens) * es;
+  const size_t budget = g_xl_budget ? g_xl_budget : d->arena.capacity() / 2;
+  auto fits = [&](size_t a, size_t b, size_t c) { return 3 * (a + b + c) <= budget; };  // slices + temporaries
+  if (fits(dsz, lsz, rsz))
+    return talshTensorContract(cptrn, dtens, ltens, rtens, sr, si, d->flat_id, DEV_NULL, COPY_TTT, accumulative,
+                               nullptr);
+  // decompose
+  std::vector<talsh_tens_op_t *> todo, ops;
+  auto release = [&]() {
+    for (auto *o : todo) talshTensorOpDestroy(o);
+    for (auto *o : ops) talshTensorOpDestroy(o);
+    todo.clear();
+    ops.clear();
+  };
+  {
+    talsh_tens_op_t *root = nullptr;
+    int e = talshTensorOpCreate(&root);
+    if (e) return e;
+    todo.push_back(root);
+    talsh_tens_t *args[3] = {dtens, ltens, rtens};
+    for (talsh_tens_t *t : args) {
+      const int r = talshTensorRank(t);
+      std::vector<size_t> z(r > 0 ? r : 1, 0);
+      e = talshTensorOpSetArgument(root, t, z.data(), t->shape_p->dims);
+      if (e) { release(); return e; }
+    }
+    e = talshTensorOpSpecify(root, TALSH_TENSOR_CONTRACT, kind, cptrn, sr, si);
+    if (e) { release(); return e; }
+  }
+  while (!todo.empty()) {
+    talsh_tens_op_t *op = todo.back();
+    todo.pop_back();
+    if (fits(slice_bytes(op->tens_slice[0], es), slice_bytes(op->tens_slice[1], es),
+             slice_bytes(op->tens_slice[2], es))) {
+      ops.push_back(op);
+      continue;
+    }
+    talsh_tens_op_t *c1 = nullptr, *c2 = nullptr;
+    int e = talshTensorOpCreate(&c1);
+    if (!e) e = talshTensorOpCreate(&c2);
+    if (!e) e = talshTensorOpDecompose2(op, c1, c2);
+    talshTensorOpDestroy(op);
+    if (c2) todo.push_back(c2);
+    if (c1) todo.push_back(c1);
+    if (e || todo.size() + ops.size() > (1u << 16)) {
+      release();
+      return e ? e : DEVICE_UNABLE;
+    }
+  }
+  // images used: a GPU image on the executing device when present, else the host image
+  const int gpu = d->flat_id;
+  talsh_tens_t *args[3] = {dtens, ltens, rtens};
+  bool on_gpu[3];
+  void *host[3];
+  for (int a = 0; a < 3; ++a) {
+    on_gpu[a] = image_body(args[a], gpu, kind) != nullptr;
+    host[a] = on_gpu[a] ? nullptr : image_body(args[a], 0, kind);
+    if (!on_gpu[a] && !host[a]) { release(); return TALSH_IN_PROGRESS; }
+  }
+  int e = talshTensorDiscardOther(dtens, on_gpu[0] ? gpu : 0, DEV_NULL);  // D's other images go stale
+  if (!e && accumulative == NOPE) {
+    if (on_gpu[0]) e = talshTensorInit(dtens, 0.0, 0.0, gpu, DEV_NULL, COPY_M, nullptr);
+    else std::memset(host[0], 0, dsz);
+  }
+  size_t stage_bytes = 0;
+  for (auto *op : ops)
+    for (int a = 0; a < 3; ++a) stage_bytes = std::max(stage_bytes, slice_bytes(op->tens_slice[a], es));
+  void *stage = nullptr;
+  if (!e && !XT_HIP_OK(hipHostMalloc(&stage, std::max<size_t>(stage_bytes, 16), hipHostMallocDefault))) e = TRY_LATER;
+  for (size_t k = 0; !e && k < ops.size(); ++k) {
+    talsh_tens_op_t *op = ops[k];
+    talsh_tens_t loc[3];
+    for (auto &t : loc) talshTensorClean(&t);
+    std::vector<int> offs[3];
+    for (int a = 0; a < 3 && !e; ++a) {
+      const talsh_tens_slice_t &sl = op->tens_slice[a];
+      const int r = sl.shape.num_dim;
+      for (int i = 0; i < r; ++i) offs[a].push_back((int)sl.bases.offsets[i]);
+      e = talshTensorConstruct(&loc[a], kind, r, sl.shape.dims, gpu, nullptr, -1, nullptr, 0.0, 0.0);
+      if (e) break;
+      // load the slice (D included: pieces accumulate into its current values)
+      if (on_gpu[a]) {
+        e = talshTensorSlice(&loc[a], args[a], offs[a].data(), gpu, DEV_NULL, COPY_MT, NOPE, nullptr);
+      } else {
+        host_box(static_cast<char *>(host[a]), args[a]->shape_p->dims, sl.bases.offsets, sl.shape.dims, r, es,
+                 static_cast<char *>(stage), true);
+        if (!XT_HIP_OK(hipMemcpy(image_body(&loc[a], gpu, kind), stage, slice_bytes(sl, es), hipMemcpyHostToDevice)))
+          e = TALSH_FAILURE;
+      }
+    }
+    if (!e)
+      e = talshTensorContract(cptrn, &loc[0], &loc[1], &loc[2], sr, si, gpu, DEV_NULL, COPY_MTT, YEP, nullptr);
+    if (!e) {  // store the D slice
+      const talsh_tens_slice_t &sl = op->tens_slice[0];
+      if (on_gpu[0]) {
+        e = talshTensorInsert(dtens, &loc[0], offs[0].data(), gpu, DEV_NULL, COPY_MT, NOPE, nullptr);
+      } else if (XT_HIP_OK(hipMemcpy(stage, image_body(&loc[0], gpu, kind), slice_bytes(sl, es),
+                                     hipMemcpyDeviceToHost))) {
+        host_box(static_cast<char *>(host[0]), dtens->shape_p->dims, sl.bases.offsets, sl.shape.dims,
+                 sl.shape.num_dim, es, static_cast<char *>(stage), false);
+      } else {
+        e = TALSH_FAILURE;
+      }
+    }
+    for (auto &t : loc)
+      if (talshTensorIsEmpty(&t) != YEP) talshTensorDestruct(&t);
+  }
+  if (stage) (void)hipHostFree(stage);
+  release();
+  return e;
+}
+
+int talshAmdSetXLBudget(size_t bytes)
+{
+  g_xl_budget = bytes;
+  return 0;
 }
 int talshTensorContractXL_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
                            double sr, double si, int dev_id, int dev_kind, int accumulative)

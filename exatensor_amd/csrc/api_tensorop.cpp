@@ -4,6 +4,7 @@
 // accounting are provided. The staged execution used by the out-of-memory
 // driver (activate / load input / execute / store output / decompose) is not
 // part of this engine yet and reports TALSH_NOT_IMPLEMENTED.
+#include <vector>
 #include <cstring>
 
 #include "common.hpp"
@@ -85,9 +86,84 @@ int talshTensorOpProgress(talsh_tens_op_t *, int *done)
   if (done) *done = NOPE;
   return TALSH_NOT_IMPLEMENTED;
 }
-int talshTensorOpDecompose2(const talsh_tens_op_t *, talsh_tens_op_t *, talsh_tens_op_t *)
+// Splits a contraction in two along one dimension, halving it (the first half
+// gets the odd element): a batch index (in L, R and D) if any is larger than 1,
+// else whichever of the right (R,D), left (L,D) and contracted (L,R) index
+// groups has the largest volume, taking that group's largest extent (the last
+// one on ties). This is the rule of TALSH/talshc.cpp:3150-3294; the children
+// carry the parent's pattern and prefactor.
+int talshTensorOpDecompose2(const talsh_tens_op_t *op, talsh_tens_op_t *c1, talsh_tens_op_t *c2)
 {
-  return TALSH_NOT_IMPLEMENTED;
+  if (!op || !c1 || !c2) return TALSH_INVALID_ARGS;
+  if (op->opkind != TALSH_TENSOR_CONTRACT) return TALSH_NOT_IMPLEMENTED;
+  if (!op->symb_pattern || op->num_args != 3) return TALSH_INVALID_ARGS;
+  int dig[2 * MAX_TENSOR_RANK], dr, lr, rr, cj;
+  int e = talsh_get_contr_ptrn_str2dig(op->symb_pattern, dig, &dr, &lr, &rr, &cj);
+  if (e) return e;
+  if (dr <= 0 && lr <= 0 && rr <= 0) return TALSH_NOT_ALLOWED;
+  if (lr + rr <= 0) return TALSH_NOT_ALLOWED;
+  const int *ldims = op->tens_slice[1].shape.dims, *rdims = op->tens_slice[2].shape.dims;
+  std::vector<int> dst(dr > 0 ? dr : 1, 0);  // D position -> 1 + L position
+  size_t lc = 1, ll = 1, lrv = 1, lb = 1, sc = 0, sl = 0, sr = 0, sb = 0;
+  int mc = -1, ml = -1, mr = -1, mb = -1;
+  for (int i = 0; i < lr; ++i) {
+    const size_t cd = (size_t)ldims[i];
+    if (dig[i] < 0) {
+      lc *= cd;
+      if (cd >= sc) { sc = cd; mc = i; }
+    } else if (dig[i] > 0) {
+      dst[dig[i] - 1] = i + 1;
+      ll *= cd;
+      if (cd >= sl) { sl = cd; ml = i; }
+    }
+  }
+  for (int j = 0; j < rr; ++j) {
+    const size_t cd = (size_t)rdims[j];
+    if (dig[lr + j] > 0) {
+      if (dst[dig[lr + j] - 1] == 0) {
+        lrv *= cd;
+        if (cd >= sr) { sr = cd; mr = lr + j; }
+      } else {  // batch index
+        ll /= cd;
+        lb *= cd;
+        if (cd >= sb) { sb = cd; mb = lr + j; }
+      }
+    }
+  }
+  if (!(lb > 1 || lrv > 1 || ll > 1 || lc > 1)) return TALSH_INVALID_REQUEST;
+  int d0 = -1, d1 = -1, d2 = -1;  // dimension to halve in D, L, R (-1: none)
+  if (lb > 1) {
+    d0 = dig[mb] - 1; d1 = dst[d0] - 1; d2 = mb - lr;
+  } else if (lrv >= ll && lrv >= lc) {
+    d0 = dig[mr] - 1; d2 = mr - lr;
+  } else if (ll >= lrv && ll >= lc) {
+    d0 = dig[ml] - 1; d1 = ml;
+  } else {
+    d1 = mc; d2 = -dig[mc] - 1;
+  }
+  const int split[3] = {d0, d1, d2};
+  for (int a = 0; a < 3 && !e; ++a) {
+    const talsh_tens_slice_t &sl = op->tens_slice[a];
+    const int rank = sl.shape.num_dim;
+    std::vector<size_t> offs(rank > 0 ? rank : 1);
+    std::vector<int> dims(rank > 0 ? rank : 1);
+    for (int i = 0; i < rank; ++i) { offs[i] = sl.bases.offsets[i]; dims[i] = sl.shape.dims[i]; }
+    const int d = split[a];
+    if (d >= 0) {
+      const int h1 = (dims[d] + 1) / 2, h2 = dims[d] - h1;
+      dims[d] = h1;
+      e = talshTensorOpSetArgument(c1, sl.tensor, offs.data(), dims.data());
+      dims[d] = h2;
+      offs[d] += (size_t)h1;
+      if (!e) e = talshTensorOpSetArgument(c2, sl.tensor, offs.data(), dims.data());
+    } else {
+      e = talshTensorOpSetArgument(c1, sl.tensor, offs.data(), dims.data());
+      if (!e) e = talshTensorOpSetArgument(c2, sl.tensor, offs.data(), dims.data());
+    }
+  }
+  if (!e) e = talshTensorOpSpecify(c1, op->opkind, op->data_kind, op->symb_pattern, op->alpha_real, op->alpha_imag);
+  if (!e) e = talshTensorOpSpecify(c2, op->opkind, op->data_kind, op->symb_pattern, op->alpha_real, op->alpha_imag);
+  return e;
 }
 
 int talshTensorOpDestruct(talsh_tens_op_t *op)

@@ -135,6 +135,11 @@ def load_library(path=LIB_PATH):
         "talshAmdBatchExecute": (C.c_int, [C.c_void_p, C.c_int]),
         "talshAmdBatchInfo": (C.c_int, [C.c_void_p, _IP, _IP, C.POINTER(C.c_double)]),
         "talshAmdBatchDestroy": (C.c_int, [C.c_void_p]),
+        "talshTensorContractXL": (C.c_int, [C.c_char_p, _TP, _TP, _TP, C.c_double, C.c_double, C.c_int, C.c_int,
+                                            C.c_int]),
+        "talshTensorSlice": (C.c_int, [_TP, _TP, _IP, C.c_int, C.c_int, C.c_int, C.c_int, _KP]),
+        "talshTensorInsert": (C.c_int, [_TP, _TP, _IP, C.c_int, C.c_int, C.c_int, C.c_int, _KP]),
+        "talshAmdSetXLBudget": (C.c_int, [C.c_size_t]),
         "talshAmdMlpCreate": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]),
         "talshAmdMlpDestroy": (C.c_int, [C.c_void_p]),
         "talshAmdMlpNumParams": (C.c_int, [C.c_void_p, _IP]),

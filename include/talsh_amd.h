@@ -111,6 +111,11 @@ int talshAmdSelectorInstall(void *mlp);
 int talshAmdSelectorChoice(const char *cptrn, int drank, const int *ddims, int lrank, const int *ldims, int rrank,
                            const int *rdims, int data_kind, int accumulative, int fast_math, int *variant, int *tile);
 
+/* Device bytes one piece of talshTensorContractXL may occupy (slices plus
+ * temporaries); contractions above it are decomposed into pieces. 0 restores
+ * the default (half of the device arena). */
+int talshAmdSetXLBudget(size_t bytes);
+
 /* Synchronize all engine streams of a GPU (kind-specific number). */
 int talshAmdDeviceSynchronize(int dev_num);
 

@@ -6,6 +6,7 @@ Run in the development container (where the reference sources exist):
     make -f oracle/ref.mk && python tests/golden/gen_golden.py
 Fixtures hold inputs and the reference's outputs only.
 """
+import ctypes as C
 import gzip
 import json
 import os
@@ -16,7 +17,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
-from oracle_lib import C4, C8, NP_DTYPE, R4, R8, Oracle, Reference  # noqa: E402
+from oracle_lib import C4, C8, NP_DTYPE, R4, R8, REF_SO, Oracle, Reference  # noqa: E402
 
 REF_TREE = "/root/reference/TALSH"
 
@@ -278,9 +279,44 @@ def slice_insert_fixtures(ref):
     print(f"slice/insert cases {len(meta)}")
 
 
+def decompose_fixtures(ref):
+    """Leaves of the reference's recursive talshTensorOpDecompose2 halving of
+    contractions (batch, right, left and contracted splits)."""
+    import tensop_ctypes as TC
+    rng = random.Random(77)
+    cases = [("D(a,b,c,d)+=L(d,i,b,j)*R(j,c,i,a)", [9, 7, 8, 5], 3), ("D(a,b,i,j)+=L(a,k,i,l)*R(l,b,k,j)", [6, 5, 7, 4], 4),
+             ("D(a,b)+=L(a,i)*R(i,b)", [37, 5], 60), ("D(a,b)+=L(i,a)*R(i,b)", [3, 4], 200),
+             ("D(a,b,c)+=L(a,c,i)*R(b,i,c)", [10, 11, 6], 13), ("D()+=L(a,b)*R(b,a)", [], 0)]
+    rows = []
+    for ci, (pat, _, _) in enumerate(cases):
+        e, dig, dr, lr, rr, cj = ref.str2dig(pat)
+        for trial in range(4):
+            ext = {}
+            lab_l = pat[pat.index("L(") + 2:pat.index(")", pat.index("L("))].split(",")
+            lab_r = pat[pat.index("R(") + 2:pat.index(")", pat.index("R("))].split(",")
+            lab_d = pat[2:pat.index(")")].split(",") if dr else []
+            for x in lab_l + lab_r + lab_d:
+                if x and x not in ext:
+                    ext[x] = rng.randint(1, 23)
+            dd = [ext[x] for x in lab_d]
+            ld = [ext[x] for x in lab_l]
+            rd = [ext[x] for x in lab_r]
+            vol = sum(int(np.prod(v)) if v else 1 for v in (dd, ld, rd))
+            limit = max(3, vol // rng.choice((2, 5, 17, 60)))
+            leaves = TC.decompose(C.CDLL(REF_SO), pat, R8, dd, ld, rd, limit)
+            rows.append(dict(pattern=pat, ddims=dd, ldims=ld, rdims=rd, limit=limit, leaves=leaves))
+    with open(os.path.join(HERE, "decompose.json"), "w") as f:
+        json.dump(dict(source="reference TAL-SH talshTensorOpDecompose2 (recursive halving)", cases=rows), f,
+                  separators=(",", ":"))
+    print(f"decompose cases {len(rows)} leaves {sum(len(r['leaves']) for r in rows)}")
+
+
 if __name__ == "__main__":
     if "--slice-insert" in sys.argv:
         slice_insert_fixtures(Reference())
+    elif "--decompose" in sys.argv:
+        decompose_fixtures(Reference())
     else:
         main()
         slice_insert_fixtures(Reference())
+        decompose_fixtures(Reference())
