@@ -245,7 +245,9 @@ def run_blocksparse(gb, reps=3):
     T = gb.T
     ext, ops = BS.generate()
     out = {"ops": len(ops), "flops": BS.flops(ext, ops)}
-    for kind in (T.R8, T.R4):
+    for kind, fast, name in ((T.R8, 0, "r8"), (T.R4, 0, "r4_fp32"), (T.R4, 1, "r4_bf16x3")):
+        if fast:  # R4 three-way BF16 split for the batch and the per-block baseline alike
+            gb.L.talshEnableFastMath(T.DEV_AMD_GPU, 0)
         tens = {}
 
         def get(tag, key, modes, init):
@@ -306,7 +308,8 @@ def run_blocksparse(gb, reps=3):
         per_block(n)
         gb.sync()
         per_block_s = time.perf_counter() - t0
-        name = "r8" if kind == T.R8 else "r4_fp32"
+        if fast:
+            gb.L.talshDisableFastMath(T.DEV_AMD_GPU, 0)
         out[name] = dict(grouped_blocks_per_s=round(n / grouped_s, 1), grouped_ms=round(grouped_s * 1e3, 3),
                          grouped_kernel_ms=round(kms / max(kn, 1), 3),
                          grouped_tflops=round(out["flops"] / grouped_s / 1e12, 3),

@@ -33,6 +33,7 @@ struct Batch {
   double alpha = 1.0;
   bool acc = true;
   int grid = 0, pack_grid = 0;
+  bool split3 = false;       // R4 under fast math: three-way BF16 split
   double flops = 0;
   talsh_dev_rsc_t mem{};     // ops + items + pack jobs + counter in the arena
   talsh_dev_rsc_t packed{};  // packed L and R blocks
@@ -46,9 +47,10 @@ struct Batch {
 
 int launch_batch(const Batch *B, hipStream_t s)
 {
-  int e = launch_pack(B->kind, B->d_jobs, B->njobs, B->nchunks, B->pack_grid, s);
+  int e = launch_pack(B->kind, B->split3, B->d_jobs, B->njobs, B->nchunks, B->pack_grid, s);
   if (e) return e;
-  return launch_grouped(B->kind, B->d_ops, B->d_items, B->nitems, B->d_counter, B->alpha, B->acc, B->grid, s);
+  return launch_grouped(B->kind, B->split3, B->d_ops, B->d_items, B->nitems, B->d_counter, B->alpha, B->acc,
+                        B->grid, s);
 }
 
 void *gpu_body(talsh_tens_t *t, int flat, int kind)
@@ -89,12 +91,15 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
   const int kind = nops > 0 && dtens[0] && dtens[0]->ndev > 0 ? dtens[0]->data_kind[0] : NO_TYPE;
   if (kind != R8 && kind != R4) return TALSH_NOT_IMPLEMENTED;
   const int es = kind_size(kind);
+  const bool split3 = kind == R4 && d->r4mode == R4Mode::kBF16x3;
+  const int kq = split3 ? 32 : 16;                 // k padding of packed operands
+  const size_t packed_es = split3 ? 3 * 2 : es;    // bytes per packed element (three bf16 planes)
   std::vector<BlockOp> ops(nops);
   std::vector<void *> dptr(nops);
   std::vector<PackJob> jobs;
-  // (operand body, role 0 = L / 1 = R) -> element offset in the packed buffer
+  // (operand body, role 0 = L / 1 = R) -> byte offset in the packed buffer
   std::map<std::pair<const void *, int>, size_t> packed_at;
-  size_t packed_elems = 0;
+  size_t packed_bytes = 0;
   auto add_job = [&](const void *src, int role, const IdxMap &rows, const IdxMap &cols, int64_t nrows, int64_t k) {
     auto it = packed_at.find({src, role});
     if (it != packed_at.end()) return it->second;
@@ -104,12 +109,12 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
     j.cols = cols;
     j.nrows = nrows;
     j.k = k;
-    j.kp = (k + 15) / 16 * 16;
-    j.dst = reinterpret_cast<void *>(packed_elems);  // offset for now, pointer after allocation
+    j.kp = (k + kq - 1) / kq * kq;
+    j.dst = reinterpret_cast<void *>(packed_bytes);  // offset for now, pointer after allocation
     jobs.push_back(j);
-    packed_at[{src, role}] = packed_elems;
-    const size_t at = packed_elems;
-    packed_elems += ((size_t)(nrows * j.kp) + 31) / 32 * 32;  // 256-byte aligned sub-buffers (R8)
+    packed_at[{src, role}] = packed_bytes;
+    const size_t at = packed_bytes;
+    packed_bytes += ((size_t)(nrows * j.kp) * packed_es + 255) / 256 * 256;  // 256-byte aligned sub-buffers
     return at;
   };
   std::vector<size_t> a_at(nops), b_at(nops);
@@ -157,7 +162,7 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
       Kr.s[q] = rs[j];
       k *= ld[i];
     }
-    const int64_t kp = (k + 15) / 16 * 16;
+    const int64_t kp = (k + kq - 1) / kq * kq;
     if (m * kp >= (1ll << 31) || n * kp >= (1ll << 31)) return TALSH_LIMIT_EXCEEDED;
     a_at[o] = add_job(pl, 0, Ml, Kl, m, k);
     b_at[o] = add_job(pr, 1, Nr, Kr, n, k);
@@ -211,18 +216,20 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
   B->alpha = alpha_re;
   B->acc = accumulative != NOPE;
   B->flops = flops;
-  B->grid = std::min(B->nitems, d->num_cus * 2);  // two resident workgroups per CU (LDS and registers)
+  B->split3 = split3;
+  // resident workgroups per CU: two (FP32/FP64 tiles), one (bf16 planes fill the LDS)
+  B->grid = std::min(B->nitems, d->num_cus * (split3 ? 1 : 2));
   B->pack_grid = (int)std::min<int64_t>(nchunks, (int64_t)d->num_cus * 8);
-  int e = image_alloc(d->flat_id, std::max<size_t>(packed_elems, 1) * es, &B->packed);
+  int e = image_alloc(d->flat_id, std::max<size_t>(packed_bytes, 256), &B->packed);
   if (e) { delete B; return e; }
   char *pk = static_cast<char *>(B->packed.gmem_p);
   std::vector<BlockOp> sorted(nops);
   for (int i = 0; i < nops; ++i) {
     sorted[i] = ops[order[i]];
-    sorted[i].a = pk + a_at[order[i]] * es;
-    sorted[i].b = pk + b_at[order[i]] * es;
+    sorted[i].a = pk + a_at[order[i]];
+    sorted[i].b = pk + b_at[order[i]];
   }
-  for (PackJob &j : jobs) j.dst = pk + reinterpret_cast<size_t>(j.dst) * es;
+  for (PackJob &j : jobs) j.dst = pk + reinterpret_cast<size_t>(j.dst);
   const size_t ob = sizeof(BlockOp) * nops, ib = sizeof(WorkItem) * sitems.size(), jb = sizeof(PackJob) * jobs.size();
   const size_t bytes = ob + ib + jb + 256;
   e = image_alloc(d->flat_id, bytes, &B->mem);

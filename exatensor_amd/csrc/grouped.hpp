@@ -17,6 +17,8 @@ struct IdxMap {
 
 // Gather of one operand block into a packed k-fast matrix:
 //   dst[r * kp + kk] = src[rows(r) + cols(kk)]  (kk < k), 0 for k <= kk < kp.
+// With the three-way BF16 split (R4 fast math) dst holds three bf16 planes of
+// nrows * kp values each.
 struct PackJob {
   const void *src;
   void *dst;
@@ -46,8 +48,11 @@ struct WorkItem {
 
 constexpr int kGroupedTile = 128;
 
-int launch_pack(int kind, const PackJob *jobs, int njobs, int64_t nchunks, int grid, hipStream_t s);
-int launch_grouped(int kind, const BlockOp *ops, const WorkItem *items, int nitems, int *counter, double alpha,
-                   bool accumulate, int grid, hipStream_t s);
+// split3: R4 operands packed as three bf16 planes and multiplied with the
+// three-way split (six MFMA products, FP64 folds); kp must then be a multiple
+// of 32.
+int launch_pack(int kind, bool split3, const PackJob *jobs, int njobs, int64_t nchunks, int grid, hipStream_t s);
+int launch_grouped(int kind, bool split3, const BlockOp *ops, const WorkItem *items, int nitems, int *counter,
+                   double alpha, bool accumulate, int grid, hipStream_t s);
 
 }  // namespace xt

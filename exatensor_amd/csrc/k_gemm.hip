@@ -15,6 +15,7 @@
 // split into real/imaginary LDS planes while staging.
 #include <algorithm>
 
+#include "bf16x3.hpp"
 #include "kernels.hpp"
 
 namespace xt {
@@ -275,38 +276,10 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
 }
 
 // ---------------------------------------------------------------------------
-// Three-way BF16 split for FP32 operands (Henry, Tang & Heinecke, "Leveraging
-// the bfloat16 artificial intelligence datatype for higher-precision
-// computations", ARITH 2019): x = b0 + b1 + b2 exactly (bf16 each, RNE);
-// x*y ~= b0c0 + (b0c1 + b1c0) + (b0c2 + b1c1 + b2c0), dropping terms below
-// 2^-24 relative. Each product runs on v_mfma_f32_16x16x32_bf16; the FP32
-// accumulator is folded into an FP64 sum every kFold k-tiles so the
-// accumulation error stays at the level of a short FP32 sum.
-typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
-typedef unsigned short us8 __attribute__((ext_vector_type(8)));
-
-__device__ __forceinline__ unsigned short bf16_rne(float x)
-{
-  uint32_t u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
-}
-__device__ __forceinline__ float bf16_to_f(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
-
-__device__ __forceinline__ void split3(float x, unsigned short &h0, unsigned short &h1, unsigned short &h2)
-{
-  h0 = bf16_rne(x);
-  const float r1 = x - bf16_to_f(h0);
-  h1 = bf16_rne(r1);
-  const float r2 = r1 - bf16_to_f(h1);
-  h2 = bf16_rne(r2);
-}
-
-__device__ __forceinline__ f4 mfma_bf(const us8 &a, const us8 &b, f4 c)
-{
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
-}
-
+// Three-way BF16 split (helpers in bf16x3.hpp): each product runs on
+// v_mfma_f32_16x16x32_bf16; the FP32 accumulator is folded into a wider sum
+// every kFold k-tiles so the accumulation error stays at the level of a short
+// FP32 sum.
 // Split pass: FP32 operand (k-fast rows x K, or row-fast) -> three bf16 planes
 // stored k-fast, zero-padded to rows_pad x k_pad (k_pad a multiple of 32), so
 // the matrix kernel needs no bounds checks on loads.

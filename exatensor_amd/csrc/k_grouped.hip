@@ -19,6 +19,7 @@
 //
 // Replaces the per-block talsh_tensor_contract calls issued by ExaTENSOR's
 // block-level caller (INTERVIRT/tavp_worker.F90:9440-9480).
+#include "bf16x3.hpp"
 #include "grouped.hpp"
 
 namespace xt {
@@ -76,7 +77,7 @@ __global__ void k_zero_counter(int *c) { *c = 0; }
 
 // Packs chunks of kPackChunk destination elements; a chunk belongs to the job
 // with the largest chunk0 not above it (binary search over the sorted jobs).
-template <typename T>
+template <typename T, bool SPLIT>
 __global__ __launch_bounds__(256) void k_pack(const PackJob *__restrict__ jobs, int njobs, int64_t nchunks)
 {
   for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
@@ -97,7 +98,16 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob *__restrict__ jobs, 
       const uint32_t r = idx / kp, kk = idx - r * kp;
       T v = T(0);
       if (kk < k) v = src[map_off(J.rows, r) + map_off(J.cols, kk)];
-      dst[idx] = v;
+      if constexpr (SPLIT) {
+        unsigned short h0, h1, h2;
+        split3(v, h0, h1, h2);
+        unsigned short *d16 = static_cast<unsigned short *>(J.dst);
+        d16[idx] = h0;
+        d16[(size_t)total + idx] = h1;
+        d16[2 * (size_t)total + idx] = h2;
+      } else {
+        dst[idx] = v;
+      }
     }
   }
 }
@@ -211,23 +221,169 @@ __global__ __launch_bounds__(256, OCC) void k_grouped_gemm(const BlockOp *__rest
   }
 }
 
+// Three-way BF16 split form of k_grouped_gemm for R4 under fast math: packed
+// operands are three bf16 planes (plane stride rows * kp); each k-tile of 32
+// runs six 16x16x32 bf16 MFMA products per output block, and the FP32
+// accumulator is folded into an FP64 sum every 8 k-tiles (the chain's k-tiles
+// count together). One workgroup per CU (double-buffered planes fill the LDS).
+__global__ __launch_bounds__(256) void k_grouped_b3(const BlockOp *__restrict__ ops, const WorkItem *__restrict__ items,
+                                                    int nitems, int *__restrict__ counter, double alpha, int accumulate)
+{
+  constexpr int BM = kGroupedTile, BN = kGroupedTile, BK = 32, KP = BK + 8;
+  constexpr int FM = BM / 32, FN = BN / 32;
+  constexpr int CA = 3 * BM * (BK / 8) / 256;  // 16-byte chunks of A (3 planes) per thread per k-tile
+  constexpr int CB = 3 * BN * (BK / 8) / 256;
+  constexpr int kFold = 8;
+  __shared__ __align__(16) unsigned short As[2][3][BM][KP];
+  __shared__ __align__(16) unsigned short Bs[2][3][BN][KP];
+  __shared__ int next_item;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * (BN / 2);
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  while (true) {
+    if (tid == 0) next_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = next_item;
+    __syncthreads();
+    if (it >= nitems) break;
+    const WorkItem w = items[it];
+    const int64_t m0 = (int64_t)w.mt * BM, n0 = (int64_t)w.nt * BN;
+    f4 acc[FM][FN];
+    double sum[FM][FN][4];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        acc[i][j] = f4{0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sum[i][j][r] = 0.0;
+      }
+    int64_t tiles_done = 0;
+    for (int oi = w.op_begin; oi < w.op_end; ++oi) {
+      const BlockOp &op = ops[oi];
+      const int64_t kp = op.kp;
+      const unsigned short *A = static_cast<const unsigned short *>(op.a);
+      const unsigned short *B = static_cast<const unsigned short *>(op.b);
+      const int64_t pa = op.m * kp, pb = op.n * kp;  // plane strides
+      us8 ra[CA], rb[CB];
+      auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int i = 0; i < CA; ++i) {
+          const int c = tid + i * 256, pl = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
+          const int64_t row = m0 + r;
+          ra[i] = row < op.m ? *reinterpret_cast<const us8 *>(A + pl * pa + row * kp + k0 + kc * 8) : us8{};
+        }
+#pragma unroll
+        for (int i = 0; i < CB; ++i) {
+          const int c = tid + i * 256, pl = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
+          const int64_t row = n0 + r;
+          rb[i] = row < op.n ? *reinterpret_cast<const us8 *>(B + pl * pb + row * kp + k0 + kc * 8) : us8{};
+        }
+      };
+      auto store = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < CA; ++i) {
+          const int c = tid + i * 256, pl = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
+          *reinterpret_cast<us8 *>(&As[buf][pl][r][kc * 8]) = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < CB; ++i) {
+          const int c = tid + i * 256, pl = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
+          *reinterpret_cast<us8 *>(&Bs[buf][pl][r][kc * 8]) = rb[i];
+        }
+      };
+      const int64_t nk = kp / BK;
+      load(0);
+      store(0);
+      __syncthreads();
+      for (int64_t kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) load((kt + 1) * BK);
+        us8 fb[3][FN];
+#pragma unroll
+        for (int pl = 0; pl < 3; ++pl)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) fb[pl][j] = *reinterpret_cast<const us8 *>(&Bs[cur][pl][wn + j * 16 + fr][fk]);
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+          us8 fa[3];
+#pragma unroll
+          for (int pl = 0; pl < 3; ++pl) fa[pl] = *reinterpret_cast<const us8 *>(&As[cur][pl][wm + i * 16 + fr][fk]);
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[0], fb[2][j], acc[i][j]);
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[1], fb[1][j], acc[i][j]);
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[2], fb[0][j], acc[i][j]);
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[0], fb[1][j], acc[i][j]);
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[1], fb[0][j], acc[i][j]);
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[0], fb[0][j], acc[i][j]);
+        }
+        if (++tiles_done % kFold == 0) {
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+#pragma unroll
+              for (int r = 0; r < 4; ++r) sum[i][j][r] += (double)acc[i][j][r];
+              acc[i][j] = f4{0, 0, 0, 0};
+            }
+        }
+        if (kt + 1 < nk) store(cur ^ 1);
+        __syncthreads();
+      }
+    }
+    const BlockOp &o0 = ops[w.op_begin];
+    float *D = static_cast<float *>(w.d);
+    int32_t coff[FN];
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int64_t col = n0 + wn + j * 16 + fr;
+      coff[j] = col < o0.n ? map_off32(o0.Nd, (uint32_t)col) : -1;
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm + i * 16 + (lane >> 4) * 4 + r;  // FP32 16x16 layout
+        if (row >= o0.m) continue;
+        const int32_t roff = map_off32(o0.Md, (uint32_t)row);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          if (coff[j] < 0) continue;
+          double v = alpha * (sum[i][j][r] + (double)acc[i][j][r]);
+          if (accumulate) v += (double)D[roff + coff[j]];
+          D[roff + coff[j]] = (float)v;
+        }
+      }
+  }
+}
+
 }  // namespace
 
-int launch_pack(int kind, const PackJob *jobs, int njobs, int64_t nchunks, int grid, hipStream_t s)
+int launch_pack(int kind, bool split3, const PackJob *jobs, int njobs, int64_t nchunks, int grid, hipStream_t s)
 {
   if (njobs <= 0 || nchunks <= 0) return 0;
-  if (kind == R8) hipLaunchKernelGGL(k_pack<double>, dim3(grid), dim3(256), 0, s, jobs, njobs, nchunks);
-  else if (kind == R4) hipLaunchKernelGGL(k_pack<float>, dim3(grid), dim3(256), 0, s, jobs, njobs, nchunks);
+  if (kind == R8 && !split3) hipLaunchKernelGGL((k_pack<double, false>), dim3(grid), dim3(256), 0, s, jobs, njobs, nchunks);
+  else if (kind == R4 && !split3) hipLaunchKernelGGL((k_pack<float, false>), dim3(grid), dim3(256), 0, s, jobs, njobs, nchunks);
+  else if (kind == R4) hipLaunchKernelGGL((k_pack<float, true>), dim3(grid), dim3(256), 0, s, jobs, njobs, nchunks);
   else return TALSH_INVALID_ARGS;
   return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
 }
 
-int launch_grouped(int kind, const BlockOp *ops, const WorkItem *items, int nitems, int *counter, double alpha,
-                   bool accumulate, int grid, hipStream_t s)
+int launch_grouped(int kind, bool split3, const BlockOp *ops, const WorkItem *items, int nitems, int *counter,
+                   double alpha, bool accumulate, int grid, hipStream_t s)
 {
   if (nitems <= 0) return 0;
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, counter);
-  if (kind == R8)
+  if (split3) {
+    if (kind != R4) return TALSH_INVALID_ARGS;
+    hipLaunchKernelGGL(k_grouped_b3, dim3(grid), dim3(256), 0, s, ops, items, nitems, counter, alpha,
+                       accumulate ? 1 : 0);
+  } else if (kind == R8)
     hipLaunchKernelGGL((k_grouped_gemm<double, 2>), dim3(grid), dim3(256), 0, s, ops, items, nitems, counter, alpha,
                        accumulate ? 1 : 0);
   else if (kind == R4)
