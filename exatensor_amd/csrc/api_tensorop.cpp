@@ -1,9 +1,10 @@
 // talsh.h tensor-operation objects (TALSH/talshc.cpp:2598-3300).
 //
-// The object lifecycle, argument/specification setters and the flop/byte
-// accounting are provided. The staged execution used by the out-of-memory
-// driver (activate / load input / execute / store output / decompose) is not
-// part of this engine yet and reports TALSH_NOT_IMPLEMENTED.
+// The object lifecycle, argument/specification setters, the flop/byte
+// accounting and the halving decomposition (used by talshTensorContractXL)
+// are provided. The staged execution API (activate / load input / execute /
+// store output / progress) reports TALSH_NOT_IMPLEMENTED: the out-of-memory
+// driver of this engine executes its pieces directly.
 #include <vector>
 #include <cstring>
 
