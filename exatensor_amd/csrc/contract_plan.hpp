@@ -34,7 +34,7 @@ int plan_contraction(const std::vector<int> &dig, int dr, int lr, int rr, const 
                      ContractPlan &P, std::vector<ContractPlan> *all);
 
 // ---- learned plan selector ---------------------------------------------------
-constexpr int kPlanFeatures = 36;
+constexpr int kPlanFeatures = 44;
 
 // One (matricization, GEMM tile) candidate and its feature vector.
 struct PlanCandidate {

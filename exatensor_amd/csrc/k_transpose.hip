@@ -511,7 +511,7 @@ int num_cus_current()
   return cus;
 }
 
-int plan(int es, int rank, const int *dims, const int *o2n, Planned &pl, int cus)
+int plan(int es, int rank, const int *dims, const int *o2n, Planned &pl, int cus, bool choose_pad = true)
 {
   pl.f = fuse_dims(rank, dims, o2n);
   const Fused &f = pl.f;
@@ -628,9 +628,9 @@ int plan(int es, int rank, const int *dims, const int *o2n, Planned &pl, int cus
   uint32_t prod = 1;
   for (int j = 0; j < P.nt && j < 3; ++j) { prod *= P.t_in[j].d; cands.push_back(prod); }
   uint32_t best = 0;
-  int bestc = conflict_cost(st, words, 0);
+  int bestc = choose_pad ? conflict_cost(st, words, 0) : 0;
   for (uint32_t c : cands) {
-    if (c == 0 || c >= P.vol_tile) continue;
+    if (!choose_pad || c == 0 || c >= P.vol_tile) continue;
     const int cc = conflict_cost(st, words, c);
     if (cc < bestc) { bestc = cc; best = c; }
   }
@@ -751,12 +751,12 @@ int dispatch(const Planned &pl, const void *in, void *out, uint64_t vol, hipStre
 
 }  // namespace
 
-int plan_permute_info(int kind, int rank, const int *dims_in, const int *o2n, PermuteInfo *info)
+int plan_permute_info(int kind, int rank, const int *dims_in, const int *o2n, PermuteInfo *info, bool with_padding)
 {
   const int es = kind_size(kind);
   if (es == 0 || rank < 0 || rank > kMaxRank) return TALSH_INVALID_ARGS;
   Planned pl;
-  plan(es, rank, dims_in, o2n, pl, 256);
+  plan(es, rank, dims_in, o2n, pl, 256, with_padding);
   info->fused_rank = pl.f.rank;
   info->path = pl.path;
   info->tile_volume = pl.path == 1 ? (int)pl.tile.vol : 0;

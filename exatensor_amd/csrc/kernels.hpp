@@ -23,7 +23,10 @@ struct PermuteInfo {
   int path;             // 0 copy, 1 tiled, 2 scatter fallback
   int vec;              // elements per global access
 };
-int plan_permute_info(int kind, int rank, const int *dims_in, const int *o2n, PermuteInfo *info);
+// with_padding = false skips the LDS padding search (the fields above do not
+// depend on it).
+int plan_permute_info(int kind, int rank, const int *dims_in, const int *o2n, PermuteInfo *info,
+                      bool with_padding = true);
 
 // ---- element-wise (k_elementwise.hip) -------------------------------------
 int launch_fill(void *d, int kind, size_t vol, double re, double im, hipStream_t s);

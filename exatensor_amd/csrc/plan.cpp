@@ -182,6 +182,15 @@ void put_perm(float *f, const std::vector<int> &dims, const std::vector<int> &o2
   f[4] = (float)fr;
 }
 
+void put_plan(float *f, int kind, const std::vector<int> &dims, const std::vector<int> &o2n)
+{
+  if (o2n.empty()) return;
+  PermuteInfo pi{};
+  if (plan_permute_info(kind, (int)dims.size(), dims.data(), o2n.data(), &pi, false)) return;
+  f[0] = (float)std::log2((double)std::max(pi.vec, 1));
+  f[1] = pi.path == 2;
+}
+
 }  // namespace
 
 int plan_candidates(const std::vector<int> &dig, int dr, int lr, int rr, const std::vector<int> &dd,
@@ -227,6 +236,15 @@ int plan_candidates(const std::vector<int> &dig, int dr, int lr, int rr, const s
       f[33] = lg2(tm * bm * tn * bn / (gm * gn));
       f[34] = (float)std::log2(tm * tn / (double)(num_cus > 0 ? num_cus : 256));
       f[35] = (float)occ;
+      // 16-byte vector loads of the GEMM operands (k-fast rows of 16-byte multiples;
+      // a_kfast / b_kfast already describe the GEMM's A and B after a swap)
+      const bool kal = ((size_t)gk * es) % 16 == 0;
+      f[36] = c.a_kfast && kal && !cplx;
+      f[37] = c.b_kfast && kal && !cplx;
+      // transpose kernel plans: vector width and the scatter fallback
+      put_plan(f + 38, kind, ld, c.l_o2n);
+      put_plan(f + 40, kind, rd, c.r_o2n);
+      if (!c.d_out_o2n.empty()) put_plan(f + 42, kind, c.dmat_dims, c.d_out_o2n);
       out.push_back(pc);
     }
   }

@@ -54,11 +54,12 @@ def test_candidate_features(lib):
 
 
 def test_model_file_roundtrip_and_host_choice(lib, tmp_path):
-    m = S.Mlp(lib, lib.talshAmdPlanFeatureCount(), 2, 32, seed=11)
+    nf = lib.talshAmdPlanFeatureCount()
+    m = S.Mlp(lib, nf, 2, 32, seed=11)
     p = m.params()
     rng = np.random.default_rng(0)
-    mean = rng.normal(size=36).astype(np.float32)
-    std = rng.uniform(0.5, 2, size=36).astype(np.float32)
+    mean = rng.normal(size=nf).astype(np.float32)
+    std = rng.uniform(0.5, 2, size=nf).astype(np.float32)
     assert lib.talshAmdMlpSetNormalization(m.h, mean.ctypes.data_as(T._FP), std.ctypes.data_as(T._FP), 1.5, 0.5) == 0
     path = str(tmp_path / "m.txt")
     m.save(path)
@@ -71,7 +72,7 @@ def test_model_file_roundtrip_and_host_choice(lib, tmp_path):
             s = S.random_contraction(rng)
             v, t, f = S.plan_features(lib, s)
             xn = (f.astype(np.float64) - mean) / std
-            pred, _ = mlp_ref.forward(p.astype(np.float64), xn, 36, 2, 32)
+            pred, _ = mlp_ref.forward(p.astype(np.float64), xn, nf, 2, 32)
             cv, ct = C.c_int(), C.c_int()
             assert lib.talshAmdSelectorChoice(s["pattern"].encode(), len(s["ddims"]), T.ints(s["ddims"]),
                                               len(s["ldims"]), T.ints(s["ldims"]), len(s["rdims"]),
@@ -137,10 +138,11 @@ def test_mlp_gradient_matches_cpu(eng, hidden):
 @pytest.mark.gpu
 def test_gpu_forward_matches_host_planner(eng):
     lib = eng.lib
-    m = S.Mlp(lib, 36, 2, 64, seed=4)
+    nf = lib.talshAmdPlanFeatureCount()
+    m = S.Mlp(lib, nf, 2, 64, seed=4)
     rng = np.random.default_rng(4)
-    mean = rng.normal(size=36).astype(np.float32)
-    std = rng.uniform(0.5, 2, size=36).astype(np.float32)
+    mean = rng.normal(size=nf).astype(np.float32)
+    std = rng.uniform(0.5, 2, size=nf).astype(np.float32)
     lib.talshAmdMlpSetNormalization(m.h, mean.ctypes.data_as(T._FP), std.ctypes.data_as(T._FP), -2.0, 3.0)
     m.install()
     try:
@@ -148,7 +150,7 @@ def test_gpu_forward_matches_host_planner(eng):
             s = S.random_contraction(rng)
             v, t, f = S.plan_features(lib, s)
             pred = m.predict(f)
-            ref, _ = mlp_ref.forward(m.params().astype(np.float64), (f.astype(np.float64) - mean) / std, 36, 2, 64)
+            ref, _ = mlp_ref.forward(m.params().astype(np.float64), (f.astype(np.float64) - mean) / std, nf, 2, 64)
             assert np.allclose(pred, ref * 3.0 - 2.0, rtol=1e-5, atol=1e-5)
             cv, ct = C.c_int(), C.c_int()
             lib.talshAmdSelectorChoice(s["pattern"].encode(), len(s["ddims"]), T.ints(s["ddims"]), len(s["ldims"]),
@@ -182,7 +184,7 @@ def test_planner_with_model_gives_same_results(eng, oracle):
     and tile shapes; results must still match the oracle."""
     from gpu_util import contract, rel_err
     lib = eng.lib
-    m = S.Mlp(lib, 36, 2, 64, seed=21)
+    m = S.Mlp(lib, lib.talshAmdPlanFeatureCount(), 2, 64, seed=21)
     m.install()
     try:
         rng = np.random.default_rng(21)
