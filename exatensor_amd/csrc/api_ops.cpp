@@ -28,11 +28,13 @@ namespace {
 
 Task *task_of(const talsh_task_t *t) { return t ? static_cast<Task *>(t->task_p) : nullptr; }
 
-int new_task(talsh_task_t *tt, Device *d, int kind, unsigned coh, int nargs)
+int new_task(talsh_task_t *tt, Device *d, int kind, unsigned coh, int nargs, hipStream_t prefer = nullptr)
 {
   Task *k = new Task();
   k->dev_flat = d->flat_id;
   k->stream = d->streams[d->next_stream.fetch_add(1) % kStreamsPerDevice];
+  for (hipStream_t st : d->streams)
+    if (prefer && st == prefer) k->stream = prefer;
   bool ok = true;
   for (hipEvent_t *e : {&k->ev_start, &k->ev_comput, &k->ev_output, &k->ev_finish, &k->ev_mm0, &k->ev_mm1})
     ok = ok && (*e = event_get(true)) != nullptr;
@@ -293,7 +295,13 @@ int run_task(talsh_task_t *user, Device *d, unsigned coh, std::vector<Arg> &args
     if (e == TRY_LATER || e == DEVICE_UNABLE) d->stats.tasks_deferred++;
     return e;
   }
-  e = new_task(tt, d, kind, coh, nargs);
+  // a blocking call runs where its output was last used, so it needs no
+  // cross-stream wait; asynchronous tasks spread over the stream pool
+  hipStream_t prefer = nullptr;
+  if (!user)
+    for (const Arg &a : args)
+      if (a.output && (prefer = image_last_stream(a.exec)) != nullptr) break;
+  e = new_task(tt, d, kind, coh, nargs, prefer);
   if (e) {
     temps.rollback();
     for (Arg &a : args) unprepare(a);
@@ -313,9 +321,13 @@ int run_task(talsh_task_t *user, Device *d, unsigned coh, std::vector<Arg> &args
   (void)hipEventRecord(k->ev_output, k->stream);
   enqueue_outputs(k, args);
   (void)hipEventRecord(k->ev_finish, k->stream);
-  for (Arg &a : args) {
-    image_touch(k->stream, a.exec);
-    if (a.src != a.exec) image_touch(k->stream, a.src);
+  {
+    std::vector<void *> imgs;
+    for (Arg &a : args) {
+      imgs.push_back(a.exec);
+      if (a.src != a.exec) imgs.push_back(a.src);
+    }
+    images_touch(k->stream, imgs.data(), (int)imgs.size());
   }
   for (auto &r : temps.bufs) k->frees.push_back(PendingFree{r.dev_id, r.buf_entry, nullptr});
   commit_args(k, args, d->flat_id);

@@ -185,8 +185,13 @@ int find_image(const talsh_tens_t *t, int dev_flat, int kind)
 size_t tensor_volume(const talsh_tens_t *t) { return tensShape_volume(t->shape_p); }
 
 // ---- ordering between tasks on different streams --------------------------------
+// device image -> event recorded after the last task that touched it, and the
+// stream it was recorded on; one event is shared by all images of a task
+// (reference counted, returned to the pool when no image refers to it)
+struct LastUse { hipEvent_t ev; hipStream_t s; };
 static std::mutex g_sync_mu;
-static std::unordered_map<void *, hipEvent_t> g_last_use;  // device image -> event of last task touching it
+static std::unordered_map<void *, LastUse> g_last_use;
+static std::unordered_map<hipEvent_t, int> g_ev_refs;
 
 // ---- event pool: hipEventCreate costs microseconds per call, and every task
 // uses several events; released events are kept per (device, timing) ------------
@@ -228,32 +233,59 @@ void event_pool_clear()
   g_evpool.clear();
 }
 
+static void ev_unref(hipEvent_t e)
+{
+  auto it = g_ev_refs.find(e);
+  if (it != g_ev_refs.end() && --it->second == 0) {
+    g_ev_refs.erase(it);
+    event_put(e, false);  // waits already enqueued on it keep their record
+  }
+}
+
 void image_wait(hipStream_t s, void *img)
 {
   std::lock_guard<std::mutex> g(g_sync_mu);
   auto it = g_last_use.find(img);
-  if (it != g_last_use.end()) (void)hipStreamWaitEvent(s, it->second, 0);
+  if (it == g_last_use.end() || it->second.s == s) return;  // same stream: queue order suffices
+  if (hipEventQuery(it->second.ev) == hipSuccess) return;    // already complete
+  (void)hipStreamWaitEvent(s, it->second.ev, 0);
 }
 
-void image_touch(hipStream_t s, void *img)
+hipStream_t image_last_stream(void *img)
 {
+  std::lock_guard<std::mutex> g(g_sync_mu);
+  auto it = g_last_use.find(img);
+  return it == g_last_use.end() ? nullptr : it->second.s;
+}
+
+void images_touch(hipStream_t s, void *const *imgs, int n)
+{
+  if (n <= 0) return;
   hipEvent_t ev = event_get(false);
   if (!ev) return;
   (void)hipEventRecord(ev, s);
   std::lock_guard<std::mutex> g(g_sync_mu);
-  auto it = g_last_use.find(img);
-  if (it != g_last_use.end()) {
-    event_put(it->second, false);  // waits already enqueued on it keep their record
-    it->second = ev;
-  } else {
-    g_last_use[img] = ev;
+  for (int i = 0; i < n; ++i) {
+    auto it = g_last_use.find(imgs[i]);
+    if (it != g_last_use.end()) {
+      if (it->second.ev == ev) continue;  // listed twice
+      ev_unref(it->second.ev);
+      it->second = LastUse{ev, s};
+    } else {
+      g_last_use[imgs[i]] = LastUse{ev, s};
+    }
+    ++g_ev_refs[ev];
   }
+  if (g_ev_refs.find(ev) == g_ev_refs.end()) event_put(ev, false);
 }
+
+void image_touch(hipStream_t s, void *img) { images_touch(s, &img, 1); }
 
 void image_forget_all()
 {
   std::lock_guard<std::mutex> g(g_sync_mu);
-  for (auto &kv : g_last_use) (void)hipEventDestroy(kv.second);
+  for (auto &kv : g_ev_refs) (void)hipEventDestroy(kv.first);
+  g_ev_refs.clear();
   g_last_use.clear();
 }
 

@@ -18,6 +18,11 @@ int find_image(const talsh_tens_t *t, int dev_flat, int kind);
 size_t tensor_volume(const talsh_tens_t *t);
 void image_wait(hipStream_t s, void *img);
 void image_touch(hipStream_t s, void *img);
+// One event recorded on s after the current work, as the last use of every
+// listed image.
+void images_touch(hipStream_t s, void *const *imgs, int n);
+// Stream of the last recorded use of an image (nullptr if none).
+hipStream_t image_last_stream(void *img);
 int prof_collect(int id, double *total_ms, long long *count);
 
 }  // namespace xt
