@@ -281,15 +281,17 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
 // every kFold k-tiles so the accumulation error stays at the level of a short
 // FP32 sum.
 // Split pass: FP32 operand (k-fast rows x K, or row-fast) -> three bf16 planes
-// stored k-fast, zero-padded to rows_pad x k_pad (k_pad a multiple of 32), so
+// stored k-fast, zero-padded to rows_pad x k_pad (k_pad a multiple of 32) in rows of
+// `pitch` elements, so
 // the matrix kernel needs no bounds checks on loads.
 template <bool KFAST>
 __global__ __launch_bounds__(256) void k_split3(const float *__restrict__ x, int64_t rows, int64_t k, int64_t ld,
-                                                 unsigned short *__restrict__ planes, int64_t rows_pad, int64_t k_pad)
+                                                 unsigned short *__restrict__ planes, int64_t rows_pad, int64_t k_pad,
+                                                 int64_t pitch)
 {
   const int64_t kc_n = k_pad / 8;
   const int64_t total = rows_pad * kc_n;
-  const int64_t plane = rows_pad * k_pad;
+  const int64_t plane = rows_pad * pitch;
   const int64_t st = (int64_t)gridDim.x * 256;
   for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += st) {
     int64_t r, kc;
@@ -305,7 +307,7 @@ __global__ __launch_bounds__(256) void k_split3(const float *__restrict__ x, int
       split3(v, a, b, c);
       h0[i] = a; h1[i] = b; h2[i] = c;
     }
-    const int64_t off = r * k_pad + kc * 8;
+    const int64_t off = r * pitch + kc * 8;
     *reinterpret_cast<us8 *>(planes + off) = h0;
     *reinterpret_cast<us8 *>(planes + plane + off) = h1;
     *reinterpret_cast<us8 *>(planes + 2 * plane + off) = h2;
@@ -313,10 +315,10 @@ __global__ __launch_bounds__(256) void k_split3(const float *__restrict__ x, int
 }
 
 struct B3Params {
-  const unsigned short *a3;  // 3 planes, rows_pad_a x k_pad each
+  const unsigned short *a3;  // 3 planes, rows_pad_a rows of `pitch` elements each
   const unsigned short *b3;
   float *c;
-  int64_t m, n, k_pad, a_rows_pad, b_rows_pad, ldc;
+  int64_t m, n, k_pad, pitch, a_rows_pad, b_rows_pad, ldc;
   double alpha;
   int beta_zero;
   int tiles_m, tiles_n;
@@ -353,18 +355,18 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const B3Params p)
   const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * (BN / 2);
-  const int64_t pa = p.a_rows_pad * p.k_pad, pb = p.b_rows_pad * p.k_pad;
+  const int64_t pa = p.a_rows_pad * p.pitch, pb = p.b_rows_pad * p.pitch;
   us8 ra[CA], rb[CB];
   auto load = [&](int64_t k0) {
 #pragma unroll
     for (int i = 0; i < CA; ++i) {
       const int c = tid + i * 256, q = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
-      ra[i] = *reinterpret_cast<const us8 *>(p.a3 + q * pa + (m0 + r) * p.k_pad + k0 + kc * 8);
+      ra[i] = *reinterpret_cast<const us8 *>(p.a3 + q * pa + (m0 + r) * p.pitch + k0 + kc * 8);
     }
 #pragma unroll
     for (int i = 0; i < CB; ++i) {
       const int c = tid + i * 256, q = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
-      rb[i] = *reinterpret_cast<const us8 *>(p.b3 + q * pb + (n0 + r) * p.k_pad + k0 + kc * 8);
+      rb[i] = *reinterpret_cast<const us8 *>(p.b3 + q * pb + (n0 + r) * p.pitch + k0 + kc * 8);
     }
   };
   auto store = [&](int buf) {
@@ -489,7 +491,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3v3(const B3Split q)
   const int64_t kend = min(kbeg + q.kchunk, p.k_pad);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 3) * 64, wn = (wave >> 2) * 64;
-  const int64_t pa = p.a_rows_pad * p.k_pad, pb = p.b_rows_pad * p.k_pad;
+  const int64_t pa = p.a_rows_pad * p.pitch, pb = p.b_rows_pad * p.pitch;
   // this lane's slot in a wave load: row (lane / 4) of 16, chunk position lane % 4
   const int lrow = lane >> 2, lpos = lane & 3;
   const int lchunk = lpos ^ ((lrow >> 2) & 3);
@@ -497,13 +499,13 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3v3(const B3Split q)
 #pragma unroll
     for (int u = 0; u < LA; ++u) {
       const int idx = wave * LA + u, pl = idx / (BM / 16), rg = idx % (BM / 16);
-      const unsigned short *g = p.a3 + pl * pa + (m0 + rg * 16 + lrow) * p.k_pad + k0 + lchunk * 8;
+      const unsigned short *g = p.a3 + pl * pa + (m0 + rg * 16 + lrow) * p.pitch + k0 + lchunk * 8;
       lds_dma16(g, &As[buf][pl][rg * 16][0]);
     }
 #pragma unroll
     for (int u = 0; u < LB; ++u) {
       const int idx = wave * LB + u, pl = idx / (BN / 16), rg = idx % (BN / 16);
-      const unsigned short *g = p.b3 + pl * pb + (n0 + rg * 16 + lrow) * p.k_pad + k0 + lchunk * 8;
+      const unsigned short *g = p.b3 + pl * pb + (n0 + rg * 16 + lrow) * p.pitch + k0 + lchunk * 8;
       lds_dma16(g, &Bs[buf][pl][rg * 16][0]);
     }
   };
@@ -608,8 +610,15 @@ int launch_family(GemmParams p, bool ak, bool bk, hipStream_t s)
 }
 
 constexpr int kB3Tile = 128;
+constexpr int64_t kKB3PitchPad = 64;
 
 int64_t round_up(int64_t x, int64_t q) { return (x + q - 1) / q * q; }
+
+// Row pitch of the split planes: rows kp elements apart with kp a multiple of
+// 256 would put the rows of one tile kp*2 bytes (a power-of-two multiple of
+// 512) apart, all on the same few memory channels; 64 elements (128 bytes) of
+// padding spread them.
+int64_t b3_pitch(int64_t kp) { return kKB3PitchPad > 0 && kp % 256 == 0 ? kp + kKB3PitchPad : kp; }
 
 int launch_bf16x3(const GemmArgs &g, void *ws, hipStream_t s)
 {
@@ -617,23 +626,24 @@ int launch_bf16x3(const GemmArgs &g, void *ws, hipStream_t s)
   const int64_t mp = round_up((int64_t)g.m, kB3M), np = round_up((int64_t)g.n, kB3N);
   if (!ws) return TALSH_INVALID_ARGS;
   unsigned short *a3 = static_cast<unsigned short *>(ws);
-  unsigned short *b3 = a3 + 3 * mp * kp;
+  const int64_t pitch = b3_pitch(kp);
+  unsigned short *b3 = a3 + 3 * mp * pitch;
   auto split = [&](const void *x, int64_t rows, bool kfast, unsigned short *out, int64_t rows_pad) {
     const int64_t total = rows_pad * (kp / 8);
     const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
     const int64_t ld = kfast ? (int64_t)g.k : rows;
     if (kfast)
       hipLaunchKernelGGL(k_split3<true>, dim3(blocks), dim3(256), 0, s, (const float *)x, rows, (int64_t)g.k, ld, out,
-                         rows_pad, kp);
+                         rows_pad, kp, pitch);
     else
       hipLaunchKernelGGL(k_split3<false>, dim3(blocks), dim3(256), 0, s, (const float *)x, rows, (int64_t)g.k, ld, out,
-                         rows_pad, kp);
+                         rows_pad, kp, pitch);
   };
   split(g.a, (int64_t)g.m, g.a_kfast, a3, mp);
   split(g.b, (int64_t)g.n, g.b_kfast, b3, np);
   B3Params p{};
   p.a3 = a3; p.b3 = b3; p.c = static_cast<float *>(g.c);
-  p.m = (int64_t)g.m; p.n = (int64_t)g.n; p.k_pad = kp; p.a_rows_pad = mp; p.b_rows_pad = np; p.ldc = (int64_t)g.m;
+  p.m = (int64_t)g.m; p.n = (int64_t)g.n; p.k_pad = kp; p.pitch = pitch; p.a_rows_pad = mp; p.b_rows_pad = np; p.ldc = (int64_t)g.m;
   p.alpha = g.alpha_re; p.beta_zero = g.beta_zero;
   p.tiles_m = (int)(mp / kB3Tile); p.tiles_n = (int)(np / kB3Tile);
   if (g.tile == 1) {  // single-pass form: FP64 folds in registers, one wave per SIMD
@@ -646,7 +656,7 @@ int launch_bf16x3(const GemmArgs &g, void *ws, hipStream_t s)
   q.p.tiles_n = (int)(np / kB3N);
   q.kchunk = std::min<int64_t>(kp, kB3Chunk);
   const int nz = (int)((kp + q.kchunk - 1) / q.kchunk);
-  q.part = reinterpret_cast<float *>(b3 + 3 * np * kp);
+  q.part = reinterpret_cast<float *>(b3 + 3 * np * pitch);
   hipLaunchKernelGGL(k_gemm_bf16x3v3, dim3(q.p.tiles_m * q.p.tiles_n, nz), dim3(512), 0, s, q);
   const int64_t mn = (int64_t)g.m * (int64_t)g.n;
   const int rb = (int)std::min<int64_t>((mn + 255) / 256, 16384);
@@ -664,7 +674,7 @@ size_t gemm_workspace_bytes(const GemmArgs &g)
   const int64_t mp = round_up((int64_t)g.m, kB3M), np = round_up((int64_t)g.n, kB3N);
   const int64_t nz = (kp + kB3Chunk - 1) / kB3Chunk;
   // split planes (bf16) + K-chunk partials (FP32, m x n each; 16-byte aligned start)
-  return (size_t)(3 * (mp + np) * kp) * sizeof(unsigned short) + (size_t)(nz * (int64_t)g.m * (int64_t)g.n) * 4 + 16;
+  return (size_t)(3 * (mp + np) * b3_pitch(kp)) * sizeof(unsigned short) + (size_t)(nz * (int64_t)g.m * (int64_t)g.n) * 4 + 16;
 }
 
 int launch_gemm(const GemmArgs &g, hipStream_t s) { return launch_gemm_ws(g, nullptr, s); }
