@@ -7,8 +7,8 @@
 //   R8 / C8 : v_mfma_f64_16x16x4_f64, FP64 accumulation;
 //   R4 / C4 : v_mfma_f32_16x16x4_f32, FP32 accumulation ("native FP32");
 //   R4 fast : three-way BF16 split on v_mfma_f32_16x16x32_bf16 (six partial
-//             products per pair, FP32 MFMA accumulation folded into FP64
-//             every few k-steps) -- see k_gemm_bf16x3 below.
+//             products per pair, two-level FP32 accumulation per K chunk,
+//             FP64 sum over the chunks) -- see k_gemm_bf16x3 below.
 // Operands: A stored k x m with k fastest ("T") or m x k with m fastest ("N");
 // B stored k x n with k fastest ("N") or n x k with n fastest ("T");
 // C is m x n column-major. Complex operands are interleaved (re, im) and are
@@ -276,22 +276,41 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
 }
 
 // ---------------------------------------------------------------------------
-// Three-way BF16 split (helpers in bf16x3.hpp): each product runs on
-// v_mfma_f32_16x16x32_bf16; the FP32 accumulator is folded into a wider sum
-// every kFold k-tiles so the accumulation error stays at the level of a short
-// FP32 sum.
-// Split pass: FP32 operand (k-fast rows x K, or row-fast) -> three bf16 planes
-// stored k-fast, zero-padded to rows_pad x k_pad (k_pad a multiple of 32) in rows of
-// `pitch` elements, so
-// the matrix kernel needs no bounds checks on loads.
-template <bool KFAST>
+// Three-way BF16 split (helpers in bf16x3.hpp): six bf16 products per k-step
+// on the bf16 MFMAs, FP32 accumulation folded every 256 k-values into a
+// second FP32 sum (two-level summation), split K over gridDim.y chunks of at
+// most kB3Chunk k-values whose partial tiles k_splitk_reduce sums in FP64.
+//
+// Operand layout ("tile-blocked"): the split pass stores each operand as three
+// bf16 planes cut into blocks of RB rows x KB k-values, 1 KB each, laid out
+// exactly as the matrix kernel's LDS stage holds them (rows of KB values,
+// 16-byte chunks swizzled against LDS bank conflicts). One wave-wide
+// global_load_lds_dwordx4 then moves one whole block, 1 KB contiguous in
+// memory, straight into LDS. (Row-major planes moved 32 or 64 bytes per row
+// per stage: partial 128-byte lines, and the memory path, not the MFMAs, set
+// the kernel time.) Blocks of one row band follow each other along k; row
+// bands are blk = RB * k_pad elements apart.
+constexpr int kB3Chunk = 4096;
+constexpr int kB3M = 256, kB3N = 128;   // output tile of one workgroup (8 waves)
+
+template <int RB, int KB>
+struct B3Layout {
+  static constexpr int kChunks = KB / 8;                  // 16-byte chunks per block row
+  static constexpr int kShift = RB == 16 ? 2 : 3;         // chunk c of row r at c ^ ((r >> kShift) & (kChunks - 1))
+  __host__ __device__ static int pos(int r, int c) { return c ^ ((r >> kShift) & (kChunks - 1)); }
+};
+
+// Split pass: FP32 operand (k-fast rows x K, or row-fast) -> three bf16
+// planes in the tile-blocked layout, zero-padded to rows_pad x k_pad, so the
+// matrix kernel needs no bounds checks.
+template <bool KFAST, int RB, int KB>
 __global__ __launch_bounds__(256) void k_split3(const float *__restrict__ x, int64_t rows, int64_t k, int64_t ld,
                                                  unsigned short *__restrict__ planes, int64_t rows_pad, int64_t k_pad,
-                                                 int64_t pitch)
+                                                 int64_t blk, int64_t plane)
 {
+  typedef B3Layout<RB, KB> Lay;
   const int64_t kc_n = k_pad / 8;
   const int64_t total = rows_pad * kc_n;
-  const int64_t plane = rows_pad * pitch;
   const int64_t st = (int64_t)gridDim.x * 256;
   for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += st) {
     int64_t r, kc;
@@ -307,7 +326,8 @@ __global__ __launch_bounds__(256) void k_split3(const float *__restrict__ x, int
       split3(v, a, b, c);
       h0[i] = a; h1[i] = b; h2[i] = c;
     }
-    const int64_t off = r * pitch + kc * 8;
+    const int rr = (int)(r % RB), c = (int)(kc % Lay::kChunks);
+    const int64_t off = (r / RB) * blk + (kc / Lay::kChunks) * (RB * KB) + rr * KB + Lay::pos(rr, c) * 8;
     *reinterpret_cast<us8 *>(planes + off) = h0;
     *reinterpret_cast<us8 *>(planes + plane + off) = h1;
     *reinterpret_cast<us8 *>(planes + 2 * plane + off) = h2;
@@ -315,261 +335,241 @@ __global__ __launch_bounds__(256) void k_split3(const float *__restrict__ x, int
 }
 
 struct B3Params {
-  const unsigned short *a3;  // 3 planes, rows_pad_a rows of `pitch` elements each
+  const unsigned short *a3;  // 3 tile-blocked planes of A, plane_a elements apart
   const unsigned short *b3;
-  float *c;
-  int64_t m, n, k_pad, pitch, a_rows_pad, b_rows_pad, ldc;
-  double alpha;
-  int beta_zero;
+  float *part;               // split-K partial C tiles, m x n column-major each
+  int64_t m, n, k_pad, kchunk;
+  int64_t blk;               // elements between row bands (both operands)
+  int64_t plane_a, plane_b;
   int tiles_m, tiles_n;
 };
 
-// The three-way split tiles (256x128, one workgroup per CU) walk columns of 4
-// M-tiles: the 32 workgroups an XCD runs at once then cover 4 x 8 tiles, the
-// arrangement that reads the fewest operand rows per k-step for 32 tiles
-// (measured: 18% faster than groups of 8, whose L2 hit rate was 40%).
-template <int G = 4>
+// The workgroups walk columns of 4 M-tiles: the 32 workgroups an XCD runs at
+// once then cover 4 x 8 tiles, the arrangement that reads the fewest operand
+// rows per k-step for 32 tiles.
 __device__ __forceinline__ void tile_of_b3(const B3Params &p, int &tm, int &tn)
 {
   GemmParams q{};
   q.tiles_m = p.tiles_m;
   q.tiles_n = p.tiles_n;
   q.xcd_remap = 1;
-  tile_of<G>(q, tm, tn);
+  tile_of<4>(q, tm, tn);
 }
-
-// Matrix kernel of the three-way split: six bf16 products per k-step, FP32
-// MFMA accumulation folded into FP64 every kFold stages.
-template <int BM, int BN>
-__global__ __launch_bounds__(256) void k_gemm_bf16x3(const B3Params p)
-{
-  constexpr int BK = 32, KP = BK + 8;
-  constexpr int FM = BM / 32, FN = BN / 32;
-  constexpr int CA = 3 * BM * (BK / 8) / 256;   // 16-byte chunks of A per thread per stage
-  constexpr int CB = 3 * BN * (BK / 8) / 256;
-  constexpr int kFold = 8;
-  __shared__ __align__(16) unsigned short As[2][3][BM][KP];
-  __shared__ __align__(16) unsigned short Bs[2][3][BN][KP];
-  int tm, tn;
-  tile_of_b3<8>(p, tm, tn);
-  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * (BN / 2);
-  const int64_t pa = p.a_rows_pad * p.pitch, pb = p.b_rows_pad * p.pitch;
-  us8 ra[CA], rb[CB];
-  auto load = [&](int64_t k0) {
-#pragma unroll
-    for (int i = 0; i < CA; ++i) {
-      const int c = tid + i * 256, q = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
-      ra[i] = *reinterpret_cast<const us8 *>(p.a3 + q * pa + (m0 + r) * p.pitch + k0 + kc * 8);
-    }
-#pragma unroll
-    for (int i = 0; i < CB; ++i) {
-      const int c = tid + i * 256, q = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
-      rb[i] = *reinterpret_cast<const us8 *>(p.b3 + q * pb + (n0 + r) * p.pitch + k0 + kc * 8);
-    }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int i = 0; i < CA; ++i) {
-      const int c = tid + i * 256, q = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
-      *reinterpret_cast<us8 *>(&As[buf][q][r][kc * 8]) = ra[i];
-    }
-#pragma unroll
-    for (int i = 0; i < CB; ++i) {
-      const int c = tid + i * 256, q = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
-      *reinterpret_cast<us8 *>(&Bs[buf][q][r][kc * 8]) = rb[i];
-    }
-  };
-  f4 acc[FM][FN];
-  double sum[FM][FN][4];
-#pragma unroll
-  for (int i = 0; i < FM; ++i)
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      acc[i][j] = f4{0, 0, 0, 0};
-#pragma unroll
-      for (int r = 0; r < 4; ++r) sum[i][j][r] = 0.0;
-    }
-  const int64_t nk = p.k_pad / BK;
-  load(0);
-  store(0);
-  __syncthreads();
-  const int fr = lane & 15, fk = (lane >> 4) * 8;
-  for (int64_t kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < nk) load((kt + 1) * BK);
-    us8 fa[3][FM], fb[3][FN];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-#pragma unroll
-      for (int i = 0; i < FM; ++i) fa[q][i] = *reinterpret_cast<const us8 *>(&As[cur][q][wm + i * 16 + fr][fk]);
-#pragma unroll
-      for (int j = 0; j < FN; ++j) fb[q][j] = *reinterpret_cast<const us8 *>(&Bs[cur][q][wn + j * 16 + fr][fk]);
-    }
-#pragma unroll
-    for (int i = 0; i < FM; ++i) {
-#pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        acc[i][j] = mfma_bf(fa[0][i], fb[2][j], acc[i][j]);
-        acc[i][j] = mfma_bf(fa[1][i], fb[1][j], acc[i][j]);
-        acc[i][j] = mfma_bf(fa[2][i], fb[0][j], acc[i][j]);
-        acc[i][j] = mfma_bf(fa[0][i], fb[1][j], acc[i][j]);
-        acc[i][j] = mfma_bf(fa[1][i], fb[0][j], acc[i][j]);
-        acc[i][j] = mfma_bf(fa[0][i], fb[0][j], acc[i][j]);
-      }
-    }
-    if ((kt + 1) % kFold == 0 || kt + 1 == nk) {
-#pragma unroll
-      for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) sum[i][j][r] += (double)acc[i][j][r];
-          acc[i][j] = f4{0, 0, 0, 0};
-        }
-    }
-    if (kt + 1 < nk) store(cur ^ 1);
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < FM; ++i)
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int64_t col = n0 + wn + j * 16 + fr;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t row = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
-        if (row < p.m && col < p.n) {
-          const int64_t off = row + col * p.ldc;
-          double v = p.alpha * sum[i][j][r];
-          if (!p.beta_zero) v += (double)p.c[off];
-          p.c[off] = (float)v;
-        }
-      }
-    }
-}
-
-// Second form of the three-way split kernel: a 256x128 tile per workgroup of
-// 8 waves (2 per SIMD, each wave 64x64 of 16x16x32 MFMAs), operands moved
-// from global memory straight into LDS (global_load_lds_dwordx4, no register
-// staging) into a double-buffered stage, and split K: gridDim.y chunks of at
-// most kB3Chunk k-values; inside a chunk the FP32 MFMA accumulator is folded
-// every kFold k-tiles into a second FP32 sum (two-level summation), and each
-// chunk's partial tile goes to a workspace that k_splitk_reduce sums in FP64.
-//
-// LDS rows hold 32 bf16 (four 16-byte chunks); chunk c of row r sits at
-// position c ^ ((r >> 2) & 3), so the 16 rows a fragment read touches at one
-// chunk index land on distinct banks.
-constexpr int kB3Chunk = 4096;
-constexpr int kB3M = 256, kB3N = 128;
-struct B3Split {
-  B3Params p;
-  float *part;       // ksplit partial C tiles, m x n column-major each (ldc = m)
-  int64_t kchunk;    // k-values per chunk (multiple of 32)
-};
 
 __device__ __forceinline__ void lds_dma16(const void *g, void *lds_wave_base)
 {
   __builtin_amdgcn_global_load_lds(g, lds_wave_base, 16, 0, 0);
 }
 
-__global__ __launch_bounds__(512, 1) void k_gemm_bf16x3v3(const B3Split q)
+// s_waitcnt with only the vector-memory count (expcnt and lgkmcnt at their
+// no-wait maxima), gfx9 encoding.
+template <int N>
+__device__ __forceinline__ void wait_vmcnt()
 {
-  constexpr int BM = kB3M, BN = kB3N, BK = 32;
-  constexpr int FM = 4, FN = 4;          // 16x16 blocks per wave (64x64)
-  constexpr int kFold = 8;
-  constexpr int LA = 3 * (BM / 16) / 8;  // wave loads of A per stage (16 rows x 64 B each)
-  constexpr int LB = 3 * (BN / 16) / 8;
-  const B3Params &p = q.p;
-  __shared__ __align__(16) unsigned short As[2][3][BM][BK];
-  __shared__ __align__(16) unsigned short Bs[2][3][BN][BK];
-  int tm, tn;
-  tile_of_b3(p, tm, tn);
-  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
-  const int64_t kbeg = (int64_t)blockIdx.y * q.kchunk;
-  const int64_t kend = min(kbeg + q.kchunk, p.k_pad);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 3) * 64, wn = (wave >> 2) * 64;
-  const int64_t pa = p.a_rows_pad * p.pitch, pb = p.b_rows_pad * p.pitch;
-  // this lane's slot in a wave load: row (lane / 4) of 16, chunk position lane % 4
-  const int lrow = lane >> 2, lpos = lane & 3;
-  const int lchunk = lpos ^ ((lrow >> 2) & 3);
-  auto issue = [&](int buf, int64_t k0) {
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+// Barrier without the release/acquire fences of __syncthreads (whose
+// workgroup-scope release waits for every outstanding load, including the
+// DMA of later stages); the callers order LDS with explicit vmcnt waits. The
+// empty asm statements keep the compiler from moving memory accesses across.
+__device__ __forceinline__ void stage_barrier()
+{
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// Issue the DMA of one k-stage: NL 1 KB blocks (A planes x row bands, then B),
+// spread over the 8 waves; block idx lands at wr + idx * 512.
+template <int RB, int KB, int NL>
+__device__ __forceinline__ void b3_issue(unsigned short *__restrict__ wr, const unsigned short *__restrict__ ga,
+                                         const unsigned short *__restrict__ gb, const B3Params &p, int64_t kb,
+                                         int wave, int lane)
+{
+  constexpr int NA = 3 * (kB3M / RB);
+  constexpr int PER = (NL + 7) / 8;
 #pragma unroll
-    for (int u = 0; u < LA; ++u) {
-      const int idx = wave * LA + u, pl = idx / (BM / 16), rg = idx % (BM / 16);
-      const unsigned short *g = p.a3 + pl * pa + (m0 + rg * 16 + lrow) * p.pitch + k0 + lchunk * 8;
-      lds_dma16(g, &As[buf][pl][rg * 16][0]);
-    }
-#pragma unroll
-    for (int u = 0; u < LB; ++u) {
-      const int idx = wave * LB + u, pl = idx / (BN / 16), rg = idx % (BN / 16);
-      const unsigned short *g = p.b3 + pl * pb + (n0 + rg * 16 + lrow) * p.pitch + k0 + lchunk * 8;
-      lds_dma16(g, &Bs[buf][pl][rg * 16][0]);
-    }
-  };
-  f4 acc[FM][FN], sum[FM][FN];
-#pragma unroll
-  for (int i = 0; i < FM; ++i)
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      acc[i][j] = f4{0, 0, 0, 0};
-      sum[i][j] = f4{0, 0, 0, 0};
-    }
-  const int64_t nk = (kend - kbeg) / BK;
-  const int fr = lane & 15, fc = lane >> 4;
-  const int fpos = (fc ^ ((fr >> 2) & 3)) * 8;  // swizzled chunk of this lane's fragment
-  issue(0, kbeg);
-  for (int64_t kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's loads of stage kt have landed in LDS
-    __syncthreads();                // everyone's have; stage kt^1 is free to refill
-    if (kt + 1 < nk) issue(cur ^ 1, kbeg + (kt + 1) * BK);
-    us8 fb[3][FN];
+  for (int u = 0; u < PER; ++u) {
+    const int idx = wave + 8 * u;
+    if (NL % 8 != 0 && u == PER - 1 && idx >= NL) break;
+    const unsigned short *g;
+    if (idx < NA) g = ga + (idx / (kB3M / RB)) * p.plane_a + (idx % (kB3M / RB)) * p.blk;
+    else g = gb + ((idx - NA) / (kB3N / RB)) * p.plane_b + ((idx - NA) % (kB3N / RB)) * p.blk;
+    lds_dma16(g + kb * (RB * KB) + lane * 8, wr + idx * 512);
+  }
+}
+
+// Form 16: k-stages of 32, 16x16x32 MFMAs, each wave 64x64 (4x4 blocks),
+// LDS double-buffered (2 x 72 KB).
+// Form 32: k-stages of 16, 32x32x16 MFMAs, each wave 64x64 (2x2 blocks),
+// four LDS stages of 36 KB, three in flight while one is multiplied.
+//
+// A stage function both issues the DMA of a later stage into `wr` and
+// multiplies the stage in `rd`. The pointers are restrict-qualified because
+// they name different stage buffers: that lets the compiler's wait-count pass
+// see that the LDS reads of `rd` do not depend on the DMA still in flight
+// into other buffers (without it every LDS read waits for all outstanding
+// loads and the stages serialise). The dependence the DMA does carry (stage
+// kt written before it is read) is enforced by the caller's vmcnt wait and
+// barrier.
+template <int FORM>
+struct B3Form;
+
+template <>
+struct B3Form<16> {
+  static constexpr int RB = 16, KB = 32, S = 2;
+  static constexpr int NL = 3 * (kB3M + kB3N) / RB;   // 72 blocks per stage, 9 per wave
+  typedef f4 Acc;
+  static constexpr int FB = 4;                        // MFMA blocks per wave side
+  __device__ static void stage(const unsigned short *__restrict__ rd, unsigned short *__restrict__ wr, bool issue,
+                               bool compute, const unsigned short *__restrict__ ga,
+                               const unsigned short *__restrict__ gb, const B3Params &p, int64_t kb, int wave,
+                               int lane, int wm, int wn, Acc (&acc)[FB][FB])
+  {
+    if (issue) b3_issue<RB, KB, NL>(wr, ga, gb, p, kb, wave, lane);
+    if (!compute) return;
+    typedef B3Layout<RB, KB> Lay;
+    const int fr = lane & 15, fc = lane >> 4;
+    const int fpos = Lay::pos(fr, fc) * 8;
+    const unsigned short *rb = rd + 3 * kB3M * KB;
+    us8 fb[3][FB];
 #pragma unroll
     for (int pl = 0; pl < 3; ++pl)
 #pragma unroll
-      for (int j = 0; j < FN; ++j) fb[pl][j] = *reinterpret_cast<const us8 *>(&Bs[cur][pl][wn + j * 16 + fr][fpos]);
+      for (int j = 0; j < FB; ++j) fb[pl][j] = *reinterpret_cast<const us8 *>(rb + (pl * kB3N + wn + j * 16 + fr) * KB + fpos);
 #pragma unroll
-    for (int i = 0; i < FM; ++i) {
+    for (int i = 0; i < FB; ++i) {
       us8 fa[3];
 #pragma unroll
-      for (int pl = 0; pl < 3; ++pl) fa[pl] = *reinterpret_cast<const us8 *>(&As[cur][pl][wm + i * 16 + fr][fpos]);
+      for (int pl = 0; pl < 3; ++pl) fa[pl] = *reinterpret_cast<const us8 *>(rd + (pl * kB3M + wm + i * 16 + fr) * KB + fpos);
       // products outer, blocks inner: consecutive MFMAs write different
-      // accumulators (smallest terms first, as in the single-pass form)
+      // accumulators; smallest terms first
 #pragma unroll
-      for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[0], fb[2][j], acc[i][j]);
+      for (int t = 0; t < 6; ++t) {
+        const int qa = t == 0 ? 0 : t == 1 ? 1 : t == 2 ? 2 : t == 3 ? 0 : t == 4 ? 1 : 0;
+        const int qb = t == 0 ? 2 : t == 1 ? 1 : t == 2 ? 0 : t == 3 ? 1 : t == 4 ? 0 : 0;
 #pragma unroll
-      for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[1], fb[1][j], acc[i][j]);
-#pragma unroll
-      for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[2], fb[0][j], acc[i][j]);
-#pragma unroll
-      for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[0], fb[1][j], acc[i][j]);
-#pragma unroll
-      for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[1], fb[0][j], acc[i][j]);
-#pragma unroll
-      for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[0], fb[0][j], acc[i][j]);
+        for (int j = 0; j < FB; ++j) acc[i][j] = mfma_bf(fa[qa], fb[qb][j], acc[i][j]);
+      }
     }
+  }
+  // item r of lane l: row 4 (l / 16) + r, column l % 16
+  __device__ static int row_of(int lane, int r) { return 4 * (lane >> 4) + r; }
+  __device__ static int col_of(int lane) { return lane & 15; }
+  static constexpr int BS = 16, NR = 4;
+};
+
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ f16v mfma_bf32(const us8 &a, const us8 &b, f16v c)
+{
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+}
+
+template <>
+struct B3Form<32> {
+  static constexpr int RB = 32, KB = 16, S = 4;
+  static constexpr int NL = 3 * (kB3M + kB3N) / RB;   // 36 blocks per stage: waves 0-3 issue 5, waves 4-7 issue 4
+  typedef f16v Acc;
+  static constexpr int FB = 2;
+  __device__ static void stage(const unsigned short *__restrict__ rd, unsigned short *__restrict__ wr, bool issue,
+                               bool compute, const unsigned short *__restrict__ ga,
+                               const unsigned short *__restrict__ gb, const B3Params &p, int64_t kb, int wave,
+                               int lane, int wm, int wn, Acc (&acc)[FB][FB])
+  {
+    if (issue) b3_issue<RB, KB, NL>(wr, ga, gb, p, kb, wave, lane);
+    if (!compute) return;
+    typedef B3Layout<RB, KB> Lay;
+    const int fr = lane & 31, fh = lane >> 5;
+    const int fpos = Lay::pos(fr, fh) * 8;
+    const unsigned short *rb = rd + 3 * kB3M * KB;
+    us8 fa[3][FB], fb[3][FB];
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl)
+#pragma unroll
+      for (int i = 0; i < FB; ++i) {
+        fa[pl][i] = *reinterpret_cast<const us8 *>(rd + (pl * kB3M + wm + i * 32 + fr) * KB + fpos);
+        fb[pl][i] = *reinterpret_cast<const us8 *>(rb + (pl * kB3N + wn + i * 32 + fr) * KB + fpos);
+      }
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+      const int qa = t == 0 ? 0 : t == 1 ? 1 : t == 2 ? 2 : t == 3 ? 0 : t == 4 ? 1 : 0;
+      const int qb = t == 0 ? 2 : t == 1 ? 1 : t == 2 ? 0 : t == 3 ? 1 : t == 4 ? 0 : 0;
+#pragma unroll
+      for (int i = 0; i < FB; ++i)
+#pragma unroll
+        for (int j = 0; j < FB; ++j) acc[i][j] = mfma_bf32(fa[qa][i], fb[qb][j], acc[i][j]);
+    }
+  }
+  // item r of lane l: row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
+  __device__ static int row_of(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+  __device__ static int col_of(int lane) { return lane & 31; }
+  static constexpr int BS = 32, NR = 16;
+};
+
+template <int FORM>
+__global__ __launch_bounds__(512, 1) void k_gemm_bf16x3(const B3Params p)
+{
+  typedef B3Form<FORM> F;
+  constexpr int S = F::S, KB = F::KB, FB = F::FB;
+  constexpr int kStage = 3 * (kB3M + kB3N) * KB;      // bf16 values per LDS stage (A planes, then B planes)
+  constexpr int kFold = 256 / KB;                     // stages per FP32 fold (256 k-values)
+  constexpr int LMIN = F::NL / 8;                     // DMA blocks every wave issues per stage (at least)
+  __shared__ __align__(16) unsigned short lds[S * kStage];
+  int tm, tn;
+  tile_of_b3(p, tm, tn);
+  const int64_t m0 = (int64_t)tm * kB3M, n0 = (int64_t)tn * kB3N;
+  const int64_t kbeg = (int64_t)blockIdx.y * p.kchunk;
+  const int64_t kend = min(kbeg + p.kchunk, p.k_pad);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 3) * 64, wn = (wave >> 2) * 64;
+  const unsigned short *ga = p.a3 + (m0 / F::RB) * p.blk, *gb = p.b3 + (n0 / F::RB) * p.blk;
+  typename F::Acc acc[FB][FB], sum[FB][FB];
+#pragma unroll
+  for (int i = 0; i < FB; ++i)
+#pragma unroll
+    for (int j = 0; j < FB; ++j) {
+      acc[i][j] = typename F::Acc{};
+      sum[i][j] = typename F::Acc{};
+    }
+  const int64_t nk = (kend - kbeg) / KB, kb0 = kbeg / KB;
+  for (int s = 0; s < S - 1; ++s)
+    if (s < nk) F::stage(lds + (S - 1) * kStage, lds + s * kStage, true, false, ga, gb, p, kb0 + s, wave, lane, wm, wn, acc);
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    const int cur = (int)(kt % S);
+    // this wave's DMA of stage kt has landed; the later stages' DMA (at
+    // least LMIN blocks per stage, in issue order) may stay in flight
+    if constexpr (S == 4) {
+      if (kt + 2 < nk) wait_vmcnt<2 * LMIN>();
+      else if (kt + 1 < nk) wait_vmcnt<LMIN>();
+      else wait_vmcnt<0>();
+    } else {
+      static_assert(S == 2, "stage count");
+      wait_vmcnt<0>();  // only stage kt is in flight here
+    }
+    stage_barrier();  // everyone's stage kt is in LDS; stage kt-1's buffer is free
+    F::stage(lds + cur * kStage, lds + ((cur + S - 1) % S) * kStage, kt + S - 1 < nk, true, ga, gb, p,
+             kb0 + kt + S - 1, wave, lane, wm, wn, acc);
     if ((kt + 1) % kFold == 0 || kt + 1 == nk) {
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+      for (int i = 0; i < FB; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) {
+        for (int j = 0; j < FB; ++j) {
           sum[i][j] += acc[i][j];
-          acc[i][j] = f4{0, 0, 0, 0};
+          acc[i][j] = typename F::Acc{};
         }
     }
   }
-  float *P = q.part + (int64_t)blockIdx.y * p.m * p.n;
+  float *P = p.part + (int64_t)blockIdx.y * p.m * p.n;
 #pragma unroll
-  for (int i = 0; i < FM; ++i)
+  for (int i = 0; i < FB; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int64_t col = n0 + wn + j * 16 + fr;
+    for (int j = 0; j < FB; ++j) {
+      const int64_t col = n0 + wn + j * F::BS + F::col_of(lane);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t row = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+      for (int r = 0; r < F::NR; ++r) {
+        const int64_t row = m0 + wm + i * F::BS + F::row_of(lane, r);
         if (row < p.m && col < p.n) P[row + col * p.m] = sum[i][j][r];
       }
     }
@@ -609,59 +609,61 @@ int launch_family(GemmParams p, bool ak, bool bk, hipStream_t s)
   return launch_grid(k_gemm_mfma<T, CPLX, false, false, BM, BN, BK, OCC>, p, s);
 }
 
-constexpr int kB3Tile = 128;
-constexpr int64_t kKB3PitchPad = 64;
+constexpr int kB3Form = 16;               // kernel form of the three-way split (16 or 32)
 
 int64_t round_up(int64_t x, int64_t q) { return (x + q - 1) / q * q; }
 
-// Row pitch of the split planes: rows kp elements apart with kp a multiple of
-// 256 would put the rows of one tile kp*2 bytes (a power-of-two multiple of
-// 512) apart, all on the same few memory channels; 64 elements (128 bytes) of
-// padding spread them.
-int64_t b3_pitch(int64_t kp) { return kKB3PitchPad > 0 && kp % 256 == 0 ? kp + kKB3PitchPad : kp; }
+struct B3Geom {
+  int64_t kp, mp, np, blk, plane_a, plane_b, nz;
+};
+
+B3Geom b3_geom(const GemmArgs &g)
+{
+  B3Geom G;
+  G.kp = round_up(std::max<int64_t>((int64_t)g.k, 1), 32);
+  G.mp = round_up((int64_t)g.m, kB3M);
+  G.np = round_up((int64_t)g.n, kB3N);
+  const int rb = B3Form<kB3Form>::RB;
+  G.blk = rb * G.kp;
+  G.plane_a = G.mp / rb * G.blk;
+  G.plane_b = G.np / rb * G.blk;
+  G.nz = (G.kp + kB3Chunk - 1) / kB3Chunk;
+  return G;
+}
 
 int launch_bf16x3(const GemmArgs &g, void *ws, hipStream_t s)
 {
-  const int64_t kp = round_up(std::max<int64_t>((int64_t)g.k, 1), 32);
-  const int64_t mp = round_up((int64_t)g.m, kB3M), np = round_up((int64_t)g.n, kB3N);
+  typedef B3Form<kB3Form> F;
   if (!ws) return TALSH_INVALID_ARGS;
+  const B3Geom G = b3_geom(g);
   unsigned short *a3 = static_cast<unsigned short *>(ws);
-  const int64_t pitch = b3_pitch(kp);
-  unsigned short *b3 = a3 + 3 * mp * pitch;
-  auto split = [&](const void *x, int64_t rows, bool kfast, unsigned short *out, int64_t rows_pad) {
-    const int64_t total = rows_pad * (kp / 8);
+  unsigned short *b3 = a3 + 3 * G.plane_a;
+  auto split = [&](const void *x, int64_t rows, bool kfast, unsigned short *out, int64_t rows_pad, int64_t plane) {
+    const int64_t total = rows_pad * (G.kp / 8);
     const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
     const int64_t ld = kfast ? (int64_t)g.k : rows;
     if (kfast)
-      hipLaunchKernelGGL(k_split3<true>, dim3(blocks), dim3(256), 0, s, (const float *)x, rows, (int64_t)g.k, ld, out,
-                         rows_pad, kp, pitch);
+      hipLaunchKernelGGL((k_split3<true, F::RB, F::KB>), dim3(blocks), dim3(256), 0, s, (const float *)x, rows,
+                         (int64_t)g.k, ld, out, rows_pad, G.kp, G.blk, plane);
     else
-      hipLaunchKernelGGL(k_split3<false>, dim3(blocks), dim3(256), 0, s, (const float *)x, rows, (int64_t)g.k, ld, out,
-                         rows_pad, kp, pitch);
+      hipLaunchKernelGGL((k_split3<false, F::RB, F::KB>), dim3(blocks), dim3(256), 0, s, (const float *)x, rows,
+                         (int64_t)g.k, ld, out, rows_pad, G.kp, G.blk, plane);
   };
-  split(g.a, (int64_t)g.m, g.a_kfast, a3, mp);
-  split(g.b, (int64_t)g.n, g.b_kfast, b3, np);
+  split(g.a, (int64_t)g.m, g.a_kfast, a3, G.mp, G.plane_a);
+  split(g.b, (int64_t)g.n, g.b_kfast, b3, G.np, G.plane_b);
   B3Params p{};
-  p.a3 = a3; p.b3 = b3; p.c = static_cast<float *>(g.c);
-  p.m = (int64_t)g.m; p.n = (int64_t)g.n; p.k_pad = kp; p.pitch = pitch; p.a_rows_pad = mp; p.b_rows_pad = np; p.ldc = (int64_t)g.m;
-  p.alpha = g.alpha_re; p.beta_zero = g.beta_zero;
-  p.tiles_m = (int)(mp / kB3Tile); p.tiles_n = (int)(np / kB3Tile);
-  if (g.tile == 1) {  // single-pass form: FP64 folds in registers, one wave per SIMD
-    hipLaunchKernelGGL((k_gemm_bf16x3<kB3Tile, kB3Tile>), dim3(p.tiles_m * p.tiles_n), dim3(256), 0, s, p);
-    return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
-  }
-  B3Split q{};
-  q.p = p;
-  q.p.tiles_m = (int)(mp / kB3M);
-  q.p.tiles_n = (int)(np / kB3N);
-  q.kchunk = std::min<int64_t>(kp, kB3Chunk);
-  const int nz = (int)((kp + q.kchunk - 1) / q.kchunk);
-  q.part = reinterpret_cast<float *>(b3 + 3 * np * pitch);
-  hipLaunchKernelGGL(k_gemm_bf16x3v3, dim3(q.p.tiles_m * q.p.tiles_n, nz), dim3(512), 0, s, q);
+  p.a3 = a3; p.b3 = b3;
+  p.part = reinterpret_cast<float *>(b3 + 3 * G.plane_b);
+  p.m = (int64_t)g.m; p.n = (int64_t)g.n; p.k_pad = G.kp;
+  p.kchunk = std::min<int64_t>(G.kp, kB3Chunk);
+  p.blk = G.blk; p.plane_a = G.plane_a; p.plane_b = G.plane_b;
+  p.tiles_m = (int)(G.mp / kB3M);
+  p.tiles_n = (int)(G.np / kB3N);
+  hipLaunchKernelGGL(k_gemm_bf16x3<kB3Form>, dim3(p.tiles_m * p.tiles_n, (int)G.nz), dim3(512), 0, s, p);
   const int64_t mn = (int64_t)g.m * (int64_t)g.n;
   const int rb = (int)std::min<int64_t>((mn + 255) / 256, 16384);
-  hipLaunchKernelGGL(k_splitk_reduce, dim3(rb), dim3(256), 0, s, q.part, nz, mn, (int64_t)g.m, p.c, p.ldc, p.alpha,
-                     p.beta_zero);
+  hipLaunchKernelGGL(k_splitk_reduce, dim3(rb), dim3(256), 0, s, p.part, (int)G.nz, mn, (int64_t)g.m,
+                     static_cast<float *>(g.c), (int64_t)g.m, g.alpha_re, g.beta_zero ? 1 : 0);
   return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
 }
 
@@ -670,11 +672,10 @@ int launch_bf16x3(const GemmArgs &g, void *ws, hipStream_t s)
 size_t gemm_workspace_bytes(const GemmArgs &g)
 {
   if (g.kind != R4 || g.r4mode != R4Mode::kBF16x3) return 0;
-  const int64_t kp = round_up(std::max<int64_t>((int64_t)g.k, 1), 32);
-  const int64_t mp = round_up((int64_t)g.m, kB3M), np = round_up((int64_t)g.n, kB3N);
-  const int64_t nz = (kp + kB3Chunk - 1) / kB3Chunk;
-  // split planes (bf16) + K-chunk partials (FP32, m x n each; 16-byte aligned start)
-  return (size_t)(3 * (mp + np) * b3_pitch(kp)) * sizeof(unsigned short) + (size_t)(nz * (int64_t)g.m * (int64_t)g.n) * 4 + 16;
+  const B3Geom G = b3_geom(g);
+  // split planes (bf16) + K-chunk partials (FP32, m x n each)
+  return (size_t)(3 * (G.plane_a + G.plane_b)) * sizeof(unsigned short) +
+         (size_t)(G.nz * (int64_t)g.m * (int64_t)g.n) * 4 + 16;
 }
 
 int launch_gemm(const GemmArgs &g, hipStream_t s) { return launch_gemm_ws(g, nullptr, s); }

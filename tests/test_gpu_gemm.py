@@ -1,6 +1,7 @@
 """Matrix kernels on awkward shapes through talshAmdGemm: the three-way BF16
-split (both kernel forms: split-K two-level and single-pass) against FP64,
-ragged m/n/k, every operand orientation, beta 0/1 and K chunk edges."""
+split (tile-blocked operands, split K) against FP64, ragged m/n/k, every
+operand orientation, beta 0/1 and K chunk edges; the tile argument must not
+change the three-way split's result."""
 import numpy as np
 import pytest
 
