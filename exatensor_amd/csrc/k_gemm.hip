@@ -290,44 +290,110 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
 // per stage: partial 128-byte lines, and the memory path, not the MFMAs, set
 // the kernel time.) Blocks of one row band follow each other along k; row
 // bands are blk = RB * k_pad elements apart.
-constexpr int kB3Chunk = 4096;
+constexpr int kB3Chunk = 8192;
 constexpr int kB3M = 256, kB3N = 128;   // output tile of one workgroup (8 waves)
 
 template <int RB, int KB>
 struct B3Layout {
   static constexpr int kChunks = KB / 8;                  // 16-byte chunks per block row
-  static constexpr int kShift = RB == 16 ? 2 : 3;         // chunk c of row r at c ^ ((r >> kShift) & (kChunks - 1))
+  // chunk c of row r sits at c ^ ((r >> kShift) & (kChunks - 1)); for the
+  // 16-row blocks, row bits 1-2 are what leaves the fragment reads free of
+  // bank conflicts (SQ_LDS_BANK_CONFLICT 0, against 4 cycles per ds_read_b128
+  // with bits 0-1, 2-3 or 3-4, and 6% more matrix throughput)
+  static constexpr int kShift = 1;
   __host__ __device__ static int pos(int r, int c) { return c ^ ((r >> kShift) & (kChunks - 1)); }
 };
 
-// Split pass: FP32 operand (k-fast rows x K, or row-fast) -> three bf16
-// planes in the tile-blocked layout, zero-padded to rows_pad x k_pad, so the
-// matrix kernel needs no bounds checks.
-template <bool KFAST, int RB, int KB>
+// Split pass: FP32 operand -> three bf16 planes in the tile-blocked layout,
+// zero-padded to rows_pad x k_pad, so the matrix kernel needs no bounds
+// checks. Both forms write whole 1 KB blocks per wave (lane l fills the
+// block's 16-byte slot l).
+//
+// k-fast source (rows of k contiguous): one wave per block, each lane reads
+// the 8 k-values of its chunk straight from its row.
+template <int RB, int KB>
 __global__ __launch_bounds__(256) void k_split3(const float *__restrict__ x, int64_t rows, int64_t k, int64_t ld,
                                                  unsigned short *__restrict__ planes, int64_t rows_pad, int64_t k_pad,
                                                  int64_t blk, int64_t plane)
 {
   typedef B3Layout<RB, KB> Lay;
-  const int64_t kc_n = k_pad / 8;
-  const int64_t total = rows_pad * kc_n;
-  const int64_t st = (int64_t)gridDim.x * 256;
-  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += st) {
-    int64_t r, kc;
-    if (KFAST) { kc = idx % kc_n; r = idx / kc_n; }
-    else { r = idx % rows_pad; kc = idx / rows_pad; }  // consecutive threads read consecutive rows
+  static_assert(RB * Lay::kChunks == 64, "one chunk per lane");
+  const int lane = threadIdx.x & 63;
+  const int64_t nkb = k_pad / KB;
+  const int64_t nblocks = rows_pad / RB * nkb;
+  const int rr = lane / Lay::kChunks;                    // slot l holds row l / 4 at position l % 4
+  const int c = Lay::pos(rr, lane % Lay::kChunks);       // (the position map is an involution in c)
+  const int64_t wst = (int64_t)gridDim.x * 4;
+  // rows that start 16-byte aligned: two 16-byte loads per lane
+  const bool vec = ((uintptr_t)x % 16 == 0) && (ld % 4 == 0);
+  for (int64_t bi = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); bi < nblocks; bi += wst) {
+    const int64_t band = bi / nkb, kb = bi % nkb;
+    const int64_t r = band * RB + rr, k0 = kb * KB + c * 8;
+    float v[8];
+    if (vec && r < rows && k0 + 8 <= k) {
+      const float4 *src = reinterpret_cast<const float4 *>(x + r * ld + k0);
+      const float4 u0 = src[0], u1 = src[1];
+      v[0] = u0.x; v[1] = u0.y; v[2] = u0.z; v[3] = u0.w; v[4] = u1.x; v[5] = u1.y; v[6] = u1.z; v[7] = u1.w;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = (r < rows && k0 + i < k) ? x[r * ld + k0 + i] : 0.f;
+    }
     us8 h0, h1, h2;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      const int64_t kk = kc * 8 + i;
-      float v = 0.f;
-      if (r < rows && kk < k) v = KFAST ? x[r * ld + kk] : x[kk * ld + r];
-      unsigned short a, b, c;
-      split3(v, a, b, c);
-      h0[i] = a; h1[i] = b; h2[i] = c;
+      unsigned short e0, e1, e2;
+      split3(v[i], e0, e1, e2);
+      h0[i] = e0; h1[i] = e1; h2[i] = e2;
     }
-    const int rr = (int)(r % RB), c = (int)(kc % Lay::kChunks);
-    const int64_t off = (r / RB) * blk + (kc / Lay::kChunks) * (RB * KB) + rr * KB + Lay::pos(rr, c) * 8;
+    const int64_t off = band * blk + kb * (RB * KB) + lane * 8;
+    *reinterpret_cast<us8 *>(planes + off) = h0;
+    *reinterpret_cast<us8 *>(planes + plane + off) = h1;
+    *reinterpret_cast<us8 *>(planes + 2 * plane + off) = h2;
+  }
+}
+
+// Row-fast source (x[kk * ld + r]): a workgroup takes 64 rows x KB k-values
+// (four blocks), reads them as KB runs of 64 contiguous floats into LDS, and
+// each wave then splits one block out of LDS.
+template <int RB, int KB>
+__global__ __launch_bounds__(256) void k_split3_rows(const float *__restrict__ x, int64_t rows, int64_t k, int64_t ld,
+                                                      unsigned short *__restrict__ planes, int64_t rows_pad,
+                                                      int64_t k_pad, int64_t blk, int64_t plane)
+{
+  typedef B3Layout<RB, KB> Lay;
+  static_assert(RB == 16 && KB == 32, "four 16-row blocks per 64-row tile, 8 threads per k-run");
+  __shared__ float t[KB][64 + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t nkb = k_pad / KB;
+  const int64_t ntiles = rows_pad / 64 * nkb;
+  const bool vec = ((uintptr_t)x % 16 == 0) && (ld % 4 == 0);
+  const int lk = tid >> 3, lr = (tid & 7) * 8;           // load: k-run lk, rows lr .. lr+7
+  const int rr = lane / Lay::kChunks;
+  const int c = Lay::pos(rr, lane % Lay::kChunks);
+  for (int64_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+    const int64_t rt = ti / nkb, kb = ti % nkb;
+    const int64_t r0 = rt * 64 + lr, kk = kb * KB + lk;
+    float v[8];
+    if (vec && kk < k && r0 + 8 <= rows) {
+      const float4 *src = reinterpret_cast<const float4 *>(x + kk * ld + r0);
+      const float4 u0 = src[0], u1 = src[1];
+      v[0] = u0.x; v[1] = u0.y; v[2] = u0.z; v[3] = u0.w; v[4] = u1.x; v[5] = u1.y; v[6] = u1.z; v[7] = u1.w;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = (kk < k && r0 + i < rows) ? x[kk * ld + r0 + i] : 0.f;
+    }
+    __syncthreads();  // the previous tile's reads of t are done
+#pragma unroll
+    for (int i = 0; i < 8; ++i) t[lk][lr + i] = v[i];
+    __syncthreads();
+    us8 h0, h1, h2;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      unsigned short e0, e1, e2;
+      split3(t[c * 8 + i][wave * 16 + rr], e0, e1, e2);
+      h0[i] = e0; h1[i] = e1; h2[i] = e2;
+    }
+    const int64_t off = (rt * 4 + wave) * blk + kb * (RB * KB) + lane * 8;
     *reinterpret_cast<us8 *>(planes + off) = h0;
     *reinterpret_cast<us8 *>(planes + plane + off) = h1;
     *reinterpret_cast<us8 *>(planes + 2 * plane + off) = h2;
@@ -401,9 +467,8 @@ __device__ __forceinline__ void b3_issue(unsigned short *__restrict__ wr, const 
 }
 
 // Form 16: k-stages of 32, 16x16x32 MFMAs, each wave 64x64 (4x4 blocks),
-// LDS double-buffered (2 x 72 KB).
-// Form 32: k-stages of 16, 32x32x16 MFMAs, each wave 64x64 (2x2 blocks),
-// four LDS stages of 36 KB, three in flight while one is multiplied.
+// LDS double-buffered (2 x 72 KB). (A form with k-stages of 16, 32x32x16
+// MFMAs and four 36 KB stages in flight measured 4% slower.)
 //
 // A stage function both issues the DMA of a later stage into `wr` and
 // multiplies the stage in `rd`. The pointers are restrict-qualified because
@@ -460,54 +525,6 @@ struct B3Form<16> {
   static constexpr int BS = 16, NR = 4;
 };
 
-typedef float f16v __attribute__((ext_vector_type(16)));
-
-__device__ __forceinline__ f16v mfma_bf32(const us8 &a, const us8 &b, f16v c)
-{
-  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
-}
-
-template <>
-struct B3Form<32> {
-  static constexpr int RB = 32, KB = 16, S = 4;
-  static constexpr int NL = 3 * (kB3M + kB3N) / RB;   // 36 blocks per stage: waves 0-3 issue 5, waves 4-7 issue 4
-  typedef f16v Acc;
-  static constexpr int FB = 2;
-  __device__ static void stage(const unsigned short *__restrict__ rd, unsigned short *__restrict__ wr, bool issue,
-                               bool compute, const unsigned short *__restrict__ ga,
-                               const unsigned short *__restrict__ gb, const B3Params &p, int64_t kb, int wave,
-                               int lane, int wm, int wn, Acc (&acc)[FB][FB])
-  {
-    if (issue) b3_issue<RB, KB, NL>(wr, ga, gb, p, kb, wave, lane);
-    if (!compute) return;
-    typedef B3Layout<RB, KB> Lay;
-    const int fr = lane & 31, fh = lane >> 5;
-    const int fpos = Lay::pos(fr, fh) * 8;
-    const unsigned short *rb = rd + 3 * kB3M * KB;
-    us8 fa[3][FB], fb[3][FB];
-#pragma unroll
-    for (int pl = 0; pl < 3; ++pl)
-#pragma unroll
-      for (int i = 0; i < FB; ++i) {
-        fa[pl][i] = *reinterpret_cast<const us8 *>(rd + (pl * kB3M + wm + i * 32 + fr) * KB + fpos);
-        fb[pl][i] = *reinterpret_cast<const us8 *>(rb + (pl * kB3N + wn + i * 32 + fr) * KB + fpos);
-      }
-#pragma unroll
-    for (int t = 0; t < 6; ++t) {
-      const int qa = t == 0 ? 0 : t == 1 ? 1 : t == 2 ? 2 : t == 3 ? 0 : t == 4 ? 1 : 0;
-      const int qb = t == 0 ? 2 : t == 1 ? 1 : t == 2 ? 0 : t == 3 ? 1 : t == 4 ? 0 : 0;
-#pragma unroll
-      for (int i = 0; i < FB; ++i)
-#pragma unroll
-        for (int j = 0; j < FB; ++j) acc[i][j] = mfma_bf32(fa[qa][i], fb[qb][j], acc[i][j]);
-    }
-  }
-  // item r of lane l: row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
-  __device__ static int row_of(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
-  __device__ static int col_of(int lane) { return lane & 31; }
-  static constexpr int BS = 32, NR = 16;
-};
-
 template <int FORM>
 __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3(const B3Params p)
 {
@@ -515,7 +532,6 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3(const B3Params p)
   constexpr int S = F::S, KB = F::KB, FB = F::FB;
   constexpr int kStage = 3 * (kB3M + kB3N) * KB;      // bf16 values per LDS stage (A planes, then B planes)
   constexpr int kFold = 256 / KB;                     // stages per FP32 fold (256 k-values)
-  constexpr int LMIN = F::NL / 8;                     // DMA blocks every wave issues per stage (at least)
   __shared__ __align__(16) unsigned short lds[S * kStage];
   int tm, tn;
   tile_of_b3(p, tm, tn);
@@ -538,16 +554,9 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3(const B3Params p)
     if (s < nk) F::stage(lds + (S - 1) * kStage, lds + s * kStage, true, false, ga, gb, p, kb0 + s, wave, lane, wm, wn, acc);
   for (int64_t kt = 0; kt < nk; ++kt) {
     const int cur = (int)(kt % S);
-    // this wave's DMA of stage kt has landed; the later stages' DMA (at
-    // least LMIN blocks per stage, in issue order) may stay in flight
-    if constexpr (S == 4) {
-      if (kt + 2 < nk) wait_vmcnt<2 * LMIN>();
-      else if (kt + 1 < nk) wait_vmcnt<LMIN>();
-      else wait_vmcnt<0>();
-    } else {
-      static_assert(S == 2, "stage count");
-      wait_vmcnt<0>();  // only stage kt is in flight here
-    }
+    // this wave's DMA of stage kt has landed (the only stage in flight here)
+    static_assert(S == 2, "stage count");
+    wait_vmcnt<0>();
     stage_barrier();  // everyone's stage kt is in LDS; stage kt-1's buffer is free
     F::stage(lds + cur * kStage, lds + ((cur + S - 1) % S) * kStage, kt + S - 1 < nk, true, ga, gb, p,
              kb0 + kt + S - 1, wave, lane, wm, wn, acc);
@@ -575,18 +584,36 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3(const B3Params p)
     }
 }
 
-// C = alpha * sum_z part[z] (+ C): FP64 sum over the K chunks.
-__global__ __launch_bounds__(256) void k_splitk_reduce(const float *__restrict__ part, int nz, int64_t mn, int64_t m,
-                                                       float *__restrict__ c, int64_t ldc, double alpha, int beta_zero)
+// C = alpha * sum_z part[z] (+ C): FP64 sum over the K chunks. C and the
+// partials are all m x n column-major with leading dimension m, so this is a
+// 1-D pass over mn values, four per thread when mn allows.
+__global__ __launch_bounds__(256) void k_splitk_reduce(const float *__restrict__ part, int nz, int64_t mn,
+                                                       float *__restrict__ c, double alpha, int beta_zero)
 {
   const int64_t st = (int64_t)gridDim.x * 256;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < mn; i += st) {
+  const int64_t n4 = (mn % 4 == 0 && ((uintptr_t)c & 15) == 0) ? mn / 4 : 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += st) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int z = 0; z < nz; ++z) {
+      const float4 v = reinterpret_cast<const float4 *>(part + (int64_t)z * mn)[i];
+      s0 += (double)v.x; s1 += (double)v.y; s2 += (double)v.z; s3 += (double)v.w;
+    }
+    float4 o;
+    if (beta_zero) {
+      o = make_float4((float)(alpha * s0), (float)(alpha * s1), (float)(alpha * s2), (float)(alpha * s3));
+    } else {
+      const float4 cv = reinterpret_cast<const float4 *>(c)[i];
+      o = make_float4((float)(alpha * s0 + (double)cv.x), (float)(alpha * s1 + (double)cv.y),
+                      (float)(alpha * s2 + (double)cv.z), (float)(alpha * s3 + (double)cv.w));
+    }
+    reinterpret_cast<float4 *>(c)[i] = o;
+  }
+  for (int64_t i = n4 * 4 + (int64_t)blockIdx.x * 256 + threadIdx.x; i < mn; i += st) {
     double s = 0.0;
     for (int z = 0; z < nz; ++z) s += (double)part[(int64_t)z * mn + i];
-    const int64_t row = i % m, col = i / m, off = row + col * ldc;
     double v = alpha * s;
-    if (!beta_zero) v += (double)c[off];
-    c[off] = (float)v;
+    if (!beta_zero) v += (double)c[i];
+    c[i] = (float)v;
   }
 }
 
@@ -609,7 +636,7 @@ int launch_family(GemmParams p, bool ak, bool bk, hipStream_t s)
   return launch_grid(k_gemm_mfma<T, CPLX, false, false, BM, BN, BK, OCC>, p, s);
 }
 
-constexpr int kB3Form = 16;               // kernel form of the three-way split (16 or 32)
+constexpr int kB3Form = 16;               // kernel form of the three-way split
 
 int64_t round_up(int64_t x, int64_t q) { return (x + q - 1) / q * q; }
 
@@ -639,15 +666,18 @@ int launch_bf16x3(const GemmArgs &g, void *ws, hipStream_t s)
   unsigned short *a3 = static_cast<unsigned short *>(ws);
   unsigned short *b3 = a3 + 3 * G.plane_a;
   auto split = [&](const void *x, int64_t rows, bool kfast, unsigned short *out, int64_t rows_pad, int64_t plane) {
-    const int64_t total = rows_pad * (G.kp / 8);
-    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
-    const int64_t ld = kfast ? (int64_t)g.k : rows;
-    if (kfast)
-      hipLaunchKernelGGL((k_split3<true, F::RB, F::KB>), dim3(blocks), dim3(256), 0, s, (const float *)x, rows,
-                         (int64_t)g.k, ld, out, rows_pad, G.kp, G.blk, plane);
-    else
-      hipLaunchKernelGGL((k_split3<false, F::RB, F::KB>), dim3(blocks), dim3(256), 0, s, (const float *)x, rows,
-                         (int64_t)g.k, ld, out, rows_pad, G.kp, G.blk, plane);
+    const float *xf = static_cast<const float *>(x);
+    if (kfast) {
+      const int64_t nblocks = rows_pad / F::RB * (G.kp / F::KB);  // one wave each, 4 per workgroup
+      const int blocks = (int)std::min<int64_t>((nblocks + 3) / 4, 65536);
+      hipLaunchKernelGGL((k_split3<F::RB, F::KB>), dim3(blocks), dim3(256), 0, s, xf, rows, (int64_t)g.k,
+                         (int64_t)g.k, out, rows_pad, G.kp, G.blk, plane);
+    } else {
+      const int64_t ntiles = rows_pad / 64 * (G.kp / F::KB);
+      const int blocks = (int)std::min<int64_t>(ntiles, 65536);
+      hipLaunchKernelGGL((k_split3_rows<F::RB, F::KB>), dim3(blocks), dim3(256), 0, s, xf, rows, (int64_t)g.k, rows,
+                         out, rows_pad, G.kp, G.blk, plane);
+    }
   };
   split(g.a, (int64_t)g.m, g.a_kfast, a3, G.mp, G.plane_a);
   split(g.b, (int64_t)g.n, g.b_kfast, b3, G.np, G.plane_b);
@@ -661,9 +691,9 @@ int launch_bf16x3(const GemmArgs &g, void *ws, hipStream_t s)
   p.tiles_n = (int)(G.np / kB3N);
   hipLaunchKernelGGL(k_gemm_bf16x3<kB3Form>, dim3(p.tiles_m * p.tiles_n, (int)G.nz), dim3(512), 0, s, p);
   const int64_t mn = (int64_t)g.m * (int64_t)g.n;
-  const int rb = (int)std::min<int64_t>((mn + 255) / 256, 16384);
-  hipLaunchKernelGGL(k_splitk_reduce, dim3(rb), dim3(256), 0, s, p.part, (int)G.nz, mn, (int64_t)g.m,
-                     static_cast<float *>(g.c), (int64_t)g.m, g.alpha_re, g.beta_zero ? 1 : 0);
+  const int rb = (int)std::min<int64_t>((mn / 4 + 255) / 256 + 1, 8192);
+  hipLaunchKernelGGL(k_splitk_reduce, dim3(rb), dim3(256), 0, s, p.part, (int)G.nz, mn, static_cast<float *>(g.c),
+                     g.alpha_re, g.beta_zero ? 1 : 0);
   return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
 }
 
