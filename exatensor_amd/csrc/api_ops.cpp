@@ -1257,25 +1257,226 @@ int talshTensorContractXL_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t 
   return talshTensorContractXL(cptrn, dtens, ltens, rtens, sr, si, dev_id, dev_kind, accumulative);
 }
 
+}  // extern "C"
+
+// ============================================================================ decompositions
+// D = L * S * R by a truncated SVD of D matricized per the pattern (open
+// indices of L x open indices of R; TALSH/talshc.cpp:5650-6058 semantics):
+// L holds the left singular vectors, R the conjugate-transposed right ones,
+// S the nv largest singular values (nv = volume of the contracted indices,
+// at most min(rows, columns)), absorbed per `absorb`. D is left unchanged
+// (the reference destroys its copy).
+static int svd_op(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                  talsh_tens_t *stens, char absorb, int dev_id, int dev_kind)
+{
+  int err = 0;
+  Device *d = exec_device(dev_id, dev_kind, &err);
+  if (!d) return err;
+  if (!cptrn || !dtens || !ltens || !rtens || !stens) return TALSH_INVALID_ARGS;
+  if (talshTensorIsEmpty(dtens) == YEP || talshTensorIsEmpty(ltens) == YEP || talshTensorIsEmpty(rtens) == YEP)
+    return TALSH_OBJECT_IS_EMPTY;
+  int mode;
+  switch (absorb) {
+  case 'N': case 'n': mode = 0; break;
+  case 'L': case 'l': mode = 1; break;
+  case 'R': case 'r': mode = 2; break;
+  case 'S': case 's': mode = 3; break;
+  default: return TALSH_INVALID_ARGS;
+  }
+  int dig[2 * MAX_TENSOR_RANK], dr, lr, rr, cj;
+  if (talsh_get_contr_ptrn_str2dig(cptrn, dig, &dr, &lr, &rr, &cj)) return TALSH_INVALID_ARGS;
+  if (dr <= 0 || lr <= 0 || rr <= 0 || cj != 0) return TALSH_INVALID_ARGS;
+  if (dr != talshTensorRank(dtens) || lr != talshTensorRank(ltens) || rr != talshTensorRank(rtens))
+    return TALSH_INVALID_ARGS;
+  int dprm[1 + MAX_TENSOR_RANK], lprm[1 + MAX_TENSOR_RANK], rprm[1 + MAX_TENSOR_RANK];
+  int ncd = 0, nlu = 0, nru = 0, nhu = 0, ierr = 0;
+  get_contraction_permutations(0, 0, lr, rr, dig, 0, dprm, lprm, rprm, &ncd, &nlu, &nru, &nhu, &ierr);
+  if (ierr || nhu != 0 || nlu <= 0 || nru <= 0 || ncd <= 0) return TALSH_INVALID_ARGS;
+  const std::vector<int> dd = dims_of(dtens), ld = dims_of(ltens), rd = dims_of(rtens);
+  // matricized orders: D (N2O) -> [L-open | R-open]; L (O2N) -> [open | contracted];
+  // R (O2N) -> [contracted | open]
+  std::vector<int> dn(dr), ln(lr), rn(rr), d_o2n(dr), l_n2o(lr), r_n2o(rr);
+  for (int i = 0; i < dr; ++i) { dn[i] = dd[dprm[1 + i] - 1]; d_o2n[dprm[1 + i] - 1] = i; }
+  for (int i = 0; i < lr; ++i) { ln[lprm[1 + i] - 1] = ld[i]; l_n2o[lprm[1 + i] - 1] = i; }
+  for (int i = 0; i < rr; ++i) { rn[rprm[1 + i] - 1] = rd[i]; r_n2o[rprm[1 + i] - 1] = i; }
+  int64_t lu = 1, ru = 1, nv = 1;
+  for (int i = 0; i < nlu; ++i) {
+    if (ln[i] != dn[i]) return TALSH_INVALID_ARGS;
+    lu *= dn[i];
+  }
+  for (int i = 0; i < nru; ++i) {
+    if (rn[ncd + i] != dn[nlu + i]) return TALSH_INVALID_ARGS;
+    ru *= dn[nlu + i];
+  }
+  std::vector<int> sdims(ncd);
+  for (int i = 0; i < ncd; ++i) {
+    if (ln[nlu + i] != rn[i]) return TALSH_INVALID_ARGS;
+    sdims[i] = rn[i];
+    nv *= rn[i];
+  }
+  if (nv > std::min(lu, ru)) return TALSH_INVALID_ARGS;  // more singular values than the rank allows
+  const int kind = tensor_kind(dtens);
+  if (tensor_kind(ltens) != kind || tensor_kind(rtens) != kind) return TALSH_INVALID_ARGS;
+  // S: constructed on the executing device when empty, reshaped to the
+  // contracted extents otherwise
+  if (talshTensorIsEmpty(stens) == YEP) {
+    const int e = talshTensorConstruct(stens, kind, ncd, sdims.data(), d->flat_id, nullptr, -1, nullptr, 0.0, 0.0);
+    if (e != TALSH_SUCCESS && e != NOT_CLEAN) return e;
+  } else {
+    if (talshTensorRank(stens) != ncd || tensor_kind(stens) != kind) return TALSH_INVALID_ARGS;
+    if (dims_of(stens) != sdims) {
+      const int e = talshTensorReshape(stens, ncd, sdims.data());
+      if (e != TALSH_SUCCESS) return e;
+    }
+  }
+  auto trivial = [](const std::vector<int> &p) {
+    for (size_t i = 0; i < p.size(); ++i)
+      if (p[i] != (int)i) return false;
+    return true;
+  };
+  std::vector<Arg> args(4);
+  args[0].t = dtens;
+  args[1].t = ltens; args[1].output = true; args[1].need_input = false;
+  args[2].t = rtens; args[2].output = true; args[2].need_input = false;
+  args[3].t = stens; args[3].output = true; args[3].need_input = false;
+  struct Body {
+    Arg *D, *L, *R, *S;
+    int kind, mode;
+    int64_t lu, ru, nv;
+    std::vector<int> dd, d_o2n, ln, l_n2o, rn, r_n2o;
+    bool dperm, lperm, rperm;
+    void *z = nullptr, *lm = nullptr, *rm = nullptr, *work = nullptr;
+    int allocate(Temps &t)
+    {
+      const size_t es = (size_t)kind_size(kind);
+      int e = 0;
+      if (dperm) e = t.alloc(es * (size_t)(lu * ru), &z);
+      if (!e && lperm) e = t.alloc(es * (size_t)(lu * nv), &lm);
+      if (!e && rperm) e = t.alloc(es * (size_t)(nv * ru), &rm);
+      if (!e) e = t.alloc(svd_workspace_bytes(kind, lu, ru), &work);
+      return e;
+    }
+    int enqueue(Task *k, Device *)
+    {
+      int e = 0;
+      if (dperm)
+        e = launch_permute(D->exec, z, kind, (int)dd.size(), dd.data(), d_o2n.data(), false, k->stream);
+      if (e) return e;
+      SvdArgs g{};
+      g.kind = kind; g.lu = lu; g.ru = ru; g.nv = nv;
+      g.a = dperm ? z : D->exec;
+      g.lmat = lperm ? lm : L->exec;
+      g.rmat = rperm ? rm : R->exec;
+      g.svals = S->exec;
+      g.work = work;
+      g.absorb = mode;
+      g.max_sweeps = 80;
+      e = launch_svd(g, k->stream);
+      if (!e && lperm) e = launch_permute(lm, L->exec, kind, (int)ln.size(), ln.data(), l_n2o.data(), false, k->stream);
+      if (!e && rperm) e = launch_permute(rm, R->exec, kind, (int)rn.size(), rn.data(), r_n2o.data(), false, k->stream);
+      return e;
+    }
+  } body{&args[0], &args[1], &args[2], &args[3], kind, mode, lu, ru, nv, dd, d_o2n, ln, l_n2o, rn, r_n2o,
+         !trivial(d_o2n), !trivial(l_n2o), !trivial(r_n2o)};
+  unsigned coh = 0;
+  for (unsigned a = 0; a < 4; ++a) argument_coherence_set_value(&coh, 4, a, COPY_M);
+  const double flops = 10.0 * (double)std::max(lu, ru) * (double)std::min(lu, ru) * (double)std::min(lu, ru);
+  return run_task(nullptr, d, coh, args, kind, flops, body);
+}
+
+extern "C" {
+
+int talshTensorDecomposeSVD(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                            talsh_tens_t *stens, const char absorb, int dev_id, int dev_kind)
+{
+  return svd_op(cptrn, dtens, ltens, rtens, stens, absorb, dev_id, dev_kind);
+}
+
+// the absorbing forms use a temporary S (TALSH/talshc.cpp:6060-6130)
+static int svd_absorbed(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens, char absorb,
+                        int dev_id, int dev_kind)
+{
+  talsh_tens_t stens;
+  int e = talshTensorClean(&stens);
+  if (e != TALSH_SUCCESS) return e;
+  e = svd_op(cptrn, dtens, ltens, rtens, &stens, absorb, dev_id, dev_kind);
+  if (talshTensorIsEmpty(&stens) != YEP) {
+    const int e2 = talshTensorDestruct(&stens);
+    if (e == TALSH_SUCCESS) e = e2;
+  }
+  return e;
+}
+int talshTensorDecomposeSVDL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                             int dev_id, int dev_kind)
+{
+  return svd_absorbed(cptrn, dtens, ltens, rtens, 'L', dev_id, dev_kind);
+}
+int talshTensorDecomposeSVDR(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                             int dev_id, int dev_kind)
+{
+  return svd_absorbed(cptrn, dtens, ltens, rtens, 'R', dev_id, dev_kind);
+}
+int talshTensorDecomposeSVDLR(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                              int dev_id, int dev_kind)
+{
+  return svd_absorbed(cptrn, dtens, ltens, rtens, 'S', dev_id, dev_kind);
+}
+
+// D := L * R with L, R the singular factors of D (singular values dropped):
+// the isometry closest to D in its matricization. The pattern has exactly one
+// contracted index; its extent is min(rows, columns) (TALSH/talshc.cpp:6111-6188).
+int talshTensorOrthogonalizeSVD(const char *cptrn, talsh_tens_t *dtens, int dev_id, int dev_kind)
+{
+  int err = 0;
+  Device *d = exec_device(dev_id, dev_kind, &err);
+  if (!d) return err;
+  if (!cptrn || !dtens) return TALSH_INVALID_ARGS;
+  if (talshTensorIsEmpty(dtens) == YEP) return TALSH_OBJECT_IS_EMPTY;
+  int dig[2 * MAX_TENSOR_RANK], dr, lr, rr, cj;
+  if (talsh_get_contr_ptrn_str2dig(cptrn, dig, &dr, &lr, &rr, &cj)) return TALSH_INVALID_ARGS;
+  if (dr <= 0 || lr <= 0 || rr <= 0 || dr != talshTensorRank(dtens)) return TALSH_INVALID_ARGS;
+  const std::vector<int> dd = dims_of(dtens);
+  std::vector<int> ldims(lr), rdims(rr);
+  int lc = -1, rc = -1;
+  int64_t lv = 1, rv = 1;
+  for (int i = 0; i < lr; ++i) {
+    if (dig[i] > 0) { ldims[i] = dd[dig[i] - 1]; lv *= ldims[i]; }
+    else if (lc < 0) lc = i;
+    else return TALSH_INVALID_ARGS;  // one contracted index only
+  }
+  for (int i = 0; i < rr; ++i) {
+    if (dig[lr + i] > 0) { rdims[i] = dd[dig[lr + i] - 1]; rv *= rdims[i]; }
+    else if (rc < 0) rc = i;
+    else return TALSH_INVALID_ARGS;
+  }
+  if (lc < 0 || rc < 0) return TALSH_INVALID_ARGS;
+  const int64_t k = std::min(lv, rv);
+  if (k > INT32_MAX) return TALSH_LIMIT_EXCEEDED;
+  ldims[lc] = rdims[rc] = (int)k;
+  const int kind = tensor_kind(dtens);
+  talsh_tens_t lt, rt, st;
+  talshTensorClean(&lt);
+  talshTensorClean(&rt);
+  talshTensorClean(&st);
+  int e = talshTensorConstruct(&lt, kind, lr, ldims.data(), d->flat_id, nullptr, -1, nullptr, 0.0, 0.0);
+  if (e == NOT_CLEAN) e = TALSH_SUCCESS;
+  if (e == TALSH_SUCCESS) {
+    e = talshTensorConstruct(&rt, kind, rr, rdims.data(), d->flat_id, nullptr, -1, nullptr, 0.0, 0.0);
+    if (e == NOT_CLEAN) e = TALSH_SUCCESS;
+    if (e == TALSH_SUCCESS) {
+      e = svd_op(cptrn, dtens, &lt, &rt, &st, 'N', dev_id, dev_kind);
+      if (e == TALSH_SUCCESS)
+        e = talshTensorContract(cptrn, dtens, &lt, &rt, 1.0, 0.0, dev_id, dev_kind, COPY_MTT, NOPE, nullptr);
+      talshTensorDestruct(&rt);
+    }
+    talshTensorDestruct(&lt);
+  }
+  if (talshTensorIsEmpty(&st) != YEP) talshTensorDestruct(&st);
+  return e;
+}
+
 // ============================================================================ not provided by this engine
-int talshTensorDecomposeSVD(const char *, talsh_tens_t *, talsh_tens_t *, talsh_tens_t *, talsh_tens_t *, const char,
-                            int, int)
-{
-  return TALSH_NOT_IMPLEMENTED;
-}
-int talshTensorDecomposeSVDL(const char *, talsh_tens_t *, talsh_tens_t *, talsh_tens_t *, int, int)
-{
-  return TALSH_NOT_IMPLEMENTED;
-}
-int talshTensorDecomposeSVDR(const char *, talsh_tens_t *, talsh_tens_t *, talsh_tens_t *, int, int)
-{
-  return TALSH_NOT_IMPLEMENTED;
-}
-int talshTensorDecomposeSVDLR(const char *, talsh_tens_t *, talsh_tens_t *, talsh_tens_t *, int, int)
-{
-  return TALSH_NOT_IMPLEMENTED;
-}
-int talshTensorOrthogonalizeSVD(const char *, talsh_tens_t *, int, int) { return TALSH_NOT_IMPLEMENTED; }
+// (nor by the reference: TALSH/talshc.cpp:6193-6204 returns TALSH_NOT_IMPLEMENTED)
 int talshTensorOrthogonalizeMGS(talsh_tens_t *, int, int *, int, int) { return TALSH_NOT_IMPLEMENTED; }
 
 }  // extern "C"

@@ -70,4 +70,23 @@ void gemm_tile_shape(int kind, R4Mode r4mode, int tile, int *bm, int *bn, int *o
 // Tile choice of launch_gemm when GemmArgs::tile < 0.
 int gemm_default_tile(int kind, R4Mode r4mode, size_t m, size_t n, size_t k);
 
+// Truncated SVD of a column-major lu x ru matrix a (tensor kind) in FP64:
+// a ~= Lmat (lu x nv) * diag(svals) * Rmat (nv x ru), the nv largest singular
+// values in descending order, Rmat holding the conjugate-transposed right
+// singular vectors; absorb: 0 none, 1 into Lmat, 2 into Rmat, 3 square roots
+// into both. Blocking on stream s (it synchronizes once per Jacobi sweep).
+// NUM_INSTABILITY if max_sweeps sweeps do not converge.
+struct SvdArgs {
+  int kind;
+  int64_t lu, ru, nv;
+  const void *a;
+  void *lmat, *rmat, *svals;
+  void *work;          // svd_workspace_bytes(kind, lu, ru)
+  int absorb;
+  int max_sweeps;
+  double *sigma_out;   // optional host copy of the min(lu, ru) singular values
+};
+int launch_svd(const SvdArgs &g, hipStream_t s);
+size_t svd_workspace_bytes(int kind, int64_t lu, int64_t ru);
+
 }  // namespace xt

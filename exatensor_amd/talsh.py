@@ -138,6 +138,11 @@ def load_library(path=LIB_PATH):
         "talshTensorContractXL": (C.c_int, [C.c_char_p, _TP, _TP, _TP, C.c_double, C.c_double, C.c_int, C.c_int,
                                             C.c_int]),
         "talshTensorSlice": (C.c_int, [_TP, _TP, _IP, C.c_int, C.c_int, C.c_int, C.c_int, _KP]),
+        "talshTensorDecomposeSVD": (C.c_int, [C.c_char_p, _TP, _TP, _TP, _TP, C.c_char, C.c_int, C.c_int]),
+        "talshTensorDecomposeSVDL": (C.c_int, [C.c_char_p, _TP, _TP, _TP, C.c_int, C.c_int]),
+        "talshTensorDecomposeSVDR": (C.c_int, [C.c_char_p, _TP, _TP, _TP, C.c_int, C.c_int]),
+        "talshTensorDecomposeSVDLR": (C.c_int, [C.c_char_p, _TP, _TP, _TP, C.c_int, C.c_int]),
+        "talshTensorOrthogonalizeSVD": (C.c_int, [C.c_char_p, _TP, C.c_int, C.c_int]),
         "talshTensorInsert": (C.c_int, [_TP, _TP, _IP, C.c_int, C.c_int, C.c_int, C.c_int, _KP]),
         "talshAmdSetXLBudget": (C.c_int, [C.c_size_t]),
         "talshAmdMlpCreate": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]),

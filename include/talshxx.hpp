@@ -13,8 +13,8 @@
  *    here live on the host and are staged per the COPY_* rules;
  *  - pinHostMemory / unpinHostMemory leave the memory pageable and return
  *    TALSH_SUCCESS (transfers from it are staged, so results are unchanged);
- *  - SVD and orthogonalisation return TALSH_NOT_IMPLEMENTED (the C-ABI's
- *    answer; the reference's GPU path has no implementation either). */
+ *  - orthogonalizeMGS returns TALSH_NOT_IMPLEMENTED (so does the
+ *    reference's C-ABI); the SVD forms run on the GPU. */
 #ifndef TALSHXX_HPP_
 #define TALSHXX_HPP_
 
