@@ -505,7 +505,11 @@ int talshTaskWait(talsh_task_t *t, int *stats)
   Task *k = task_of(t);
   if (!k) { if (stats) *stats = TALSH_TASK_EMPTY; return TALSH_OBJECT_IS_EMPTY; }
   if (!k->finalized) {
-    const hipError_t e = hipEventSynchronize(k->ev_finish);
+    // polls the completion event, as the reference's wait loops on
+    // talshTaskComplete (TALSH/talshc.cpp:2481): a blocking event wait adds
+    // the runtime's wake-up latency to every synchronous call
+    hipError_t e;
+    while ((e = hipEventQuery(k->ev_finish)) == hipErrorNotReady) {}
     task_finalize(t, e == hipSuccess ? TALSH_TASK_COMPLETED : TALSH_TASK_ERROR);
   }
   if (stats) *stats = k->status;

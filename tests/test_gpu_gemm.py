@@ -47,6 +47,8 @@ def _gemm(m, n, k, ak, bk, beta_zero, tile, fast, kind=T.R4, seed=0):
     (200, 130, 4500, 1, 0, True),
     (513, 255, 2048, 0, 0, False),
     (1000, 700, 6200, 1, 1, True),
+    (264, 136, 8500, 1, 1, False),
+    (130, 70, 17000, 0, 1, True),
 ])
 def test_bf16x3_shapes_against_fp64(m, n, k, ak, bk, beta_zero, tile):
     e, got, want = _gemm(m, n, k, ak, bk, beta_zero, tile, fast=1)
