@@ -13,10 +13,11 @@
 //    group may be split; ragged splits are masked);
 //  * LDS padding is chosen on the host by simulating the bank conflicts of the
 //    store phase for each candidate padding period;
-//  * persistent kernel: each thread decodes the intra-tile offsets of its
-//    elements once (input offset, output offset, LDS slots) and keeps them in
-//    registers, so the per-tile work is one base-offset decode per workgroup
-//    plus one coalesced load and one coalesced store per element.
+//  * persistent kernel (four resident workgroups per CU): each thread decodes
+//    the intra-tile offsets of its vectors once (input offset, output offset,
+//    LDS read base) and keeps them in registers, so the per-tile work is one
+//    base-offset decode per workgroup plus one coalesced load and one
+//    coalesced store per vector.
 #include <algorithm>
 #include <mutex>
 #include <string>
@@ -33,6 +34,7 @@ constexpr int kNT = 256;           // threads per workgroup
 constexpr int kMaxTileDims = 8;    // dimensions inside one tile
 constexpr int kMaxGridDims = 16;   // fused rank handled by the tiled path
 constexpr int kMaxEPT = 16;        // tile elements per thread
+constexpr int kTiledPerCU = 4;     // persistent workgroups per CU
 
 struct DevDiv { uint32_t d, m, s; };
 
@@ -83,6 +85,11 @@ __device__ __forceinline__ uint4 conj_if<uint4, true>(uint4 v) { v.w ^= 0x800000
 template <typename W, int VEC>
 struct __align__(sizeof(W) * VEC) Pack { W w[VEC]; };
 
+// Hides a value's origin from the optimizer, so that addresses derived from it
+// are recomputed where they are used instead of being hoisted out of the tile
+// loop and held in registers for every element.
+__device__ __forceinline__ void opaque(uint32_t &x) { asm volatile("" : "+v"(x)); }
+
 // Streaming (nontemporal) global accesses: every element is read once and
 // written once, so the data should not displace useful lines from L2/MALL.
 typedef unsigned int nt1 __attribute__((ext_vector_type(1)));
@@ -115,6 +122,12 @@ __device__ __forceinline__ void st_stream(P *p, const P &v)
 // VEC consecutive elements along the input-minor dimension for the load and
 // along the output-minor dimension for the store, so global accesses are
 // 16-byte wide whenever both minor extents allow it.
+//
+// Register budget: a thread keeps per vector only its input offset, output
+// offset and LDS read base (plus the ragged-edge coordinates); the LDS slots
+// are recomputed per tile from those (a few integer operations) instead of
+// being held for every element, which keeps the common instantiations at
+// 30-60 VGPRs, i.e. at full occupancy.
 template <typename W, int VEC, int EPT, bool CONJ, typename OffT, bool RAG>
 __global__ __launch_bounds__(kNT) void k_permute_tiled(const W *__restrict__ in, W *__restrict__ out,
                                                         const TileParams p)
@@ -170,86 +183,95 @@ __global__ __launch_bounds__(kNT) void k_permute_tiled(const W *__restrict__ in,
     }
   }
   const uint32_t step = p.ls_out[0];
+  struct TileBase { OffT bin, bout; uint32_t rem0, rem1; bool full; };
   // tile base decode (uniform across the workgroup)
-  auto decode = [&](uint32_t tile, OffT &bin, OffT &bout, uint32_t *rem, bool &full) {
-    bin = 0;
-    bout = 0;
-    rem[0] = rem[1] = 0xFFFFu;
+  auto decode = [&](uint32_t tile, TileBase &s) {
+    s.bin = 0;
+    s.bout = 0;
+    uint32_t rem[2] = {0xFFFFu, 0xFFFFu};
     uint32_t r = tile;
     for (int g = 0; g < p.ng; ++g) {
       const uint32_t q = fdiv(r, p.gdiv[g]);
       const uint32_t c = r - q * p.gdiv[g].d;
       r = q;
-      bin += (OffT)c * (OffT)p.g_is[g];
-      bout += (OffT)c * (OffT)p.g_os[g];
+      s.bin += (OffT)c * (OffT)p.g_is[g];
+      s.bout += (OffT)c * (OffT)p.g_os[g];
       if (RAG && p.g_rag[g] >= 0) {
         const uint32_t left = p.g_ext[g] - c * p.g_t[g];
         rem[p.g_rag[g]] = left < p.g_t[g] ? left : p.g_t[g];
       }
     }
-    full = !RAG || ((rem[0] == p.rag_t[0]) && (p.nrag < 2 || rem[1] == p.rag_t[1]));
+    s.rem0 = rem[0];
+    s.rem1 = rem[1];
+    s.full = !RAG || ((rem[0] == p.rag_t[0]) && (p.nrag < 2 || rem[1] == p.rag_t[1]));
   };
   P val[EPT];
-  auto load = [&](OffT bin, const uint32_t *rem, bool full) {
+  auto load = [&](const TileBase &s) {
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const uint32_t v = tid + (uint32_t)k * kNT;
       bool ok = v < nvec;
-      if (RAG && !full) ok = ok && (rin[k] & 0xFFFFu) < rem[0] && (rin[k] >> 16) < rem[1];
-      if (ok) val[k] = ld_stream(reinterpret_cast<const P *>(in + (bin + in_off[k])));
+      if (RAG && !s.full) ok = ok && (rin[k] & 0xFFFFu) < s.rem0 && (rin[k] >> 16) < s.rem1;
+      if (ok) val[k] = ld_stream(reinterpret_cast<const P *>(in + (s.bin + in_off[k])));
     }
   };
-  uint32_t tile = blockIdx.x;
-  if (tile >= p.ntiles) return;
-  OffT bin, bout;
-  uint32_t rem[2];
-  bool full;
-  decode(tile, bin, bout, rem, full);
-  load(bin, rem, full);
-  // software pipeline: the next tile's loads are in flight while the current
-  // tile drains from LDS to memory
-  while (true) {
+  // the padding period is a multiple of VEC (checked by the planner), so the
+  // VEC slots of one loaded vector are consecutive: base slot + j
+  auto to_lds = [&](const TileBase &s) {
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
-      const uint32_t v = tid + (uint32_t)k * kNT;
+      uint32_t v = tid + (uint32_t)k * kNT;
       bool ok = v < nvec;
-      if (RAG && !full) ok = ok && (rin[k] & 0xFFFFu) < rem[0] && (rin[k] >> 16) < rem[1];
+      if (RAG && !s.full) ok = ok && (rin[k] & 0xFFFFu) < s.rem0 && (rin[k] >> 16) < s.rem1;
       if (ok) {
+        opaque(v);
+        const uint32_t e0 = v * VEC;
+        const uint32_t b = e0 + fdiv(e0, p.pad);
 #pragma unroll
-        for (int j = 0; j < VEC; ++j) {
-          const uint32_t e = v * VEC + j;
-          lds[e + fdiv(e, p.pad)] = val[k].w[j];
-        }
+        for (int j = 0; j < VEC; ++j) lds[b + j] = val[k].w[j];
       }
     }
-    __syncthreads();
-    const uint32_t next = tile + gridDim.x;
-    OffT nbin = 0, nbout = 0;
-    uint32_t nrem[2] = {0xFFFFu, 0xFFFFu};
-    bool nfull = true;
-    if (next < p.ntiles) {
-      decode(next, nbin, nbout, nrem, nfull);
-      load(nbin, nrem, nfull);
-    }
+  };
+  auto from_lds = [&](const TileBase &s) {
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const uint32_t f = tid + (uint32_t)k * kNT;
       bool ok = f < nvec;
-      if (RAG && !full) ok = ok && (rout[k] & 0xFFFFu) < rem[0] && (rout[k] >> 16) < rem[1];
+      if (RAG && !s.full) ok = ok && (rout[k] & 0xFFFFu) < s.rem0 && (rout[k] >> 16) < s.rem1;
       if (ok) {
+        uint32_t base = st_ls[k];
+        opaque(base);
         P o;
 #pragma unroll
         for (int j = 0; j < VEC; ++j) {
-          const uint32_t e = st_ls[k] + j * step;
+          const uint32_t e = base + j * step;
           o.w[j] = conj_if<W, CONJ>(lds[e + fdiv(e, p.pad)]);
         }
-        st_stream(reinterpret_cast<P *>(out + (bout + out_off[k])), o);
+        st_stream(reinterpret_cast<P *>(out + (s.bout + out_off[k])), o);
       }
     }
+  };
+  uint32_t tile = blockIdx.x;
+  if (tile >= p.ntiles) return;
+  TileBase cur;
+  decode(tile, cur);
+  load(cur);
+  // software pipeline: the next tile's loads are in flight while the current
+  // tile drains from LDS to memory
+  while (true) {
+    to_lds(cur);
+    __syncthreads();
+    const uint32_t next = tile + gridDim.x;
+    TileBase nxt;
+    if (next < p.ntiles) {
+      decode(next, nxt);
+      load(nxt);
+    }
+    from_lds(cur);
     if (next >= p.ntiles) break;
     __syncthreads();
     tile = next;
-    bin = nbin; bout = nbout; rem[0] = nrem[0]; rem[1] = nrem[1]; full = nfull;
+    cur = nxt;
   }
 }
 
@@ -498,6 +520,7 @@ struct Planned {
   int vec = 1;        // elements per global access
   bool wide = false;  // 64-bit element offsets
   bool rag = false;   // masked edge tiles
+  int cus = 0;        // compute units the grid was sized for
   TilePlan tile;
 };
 
@@ -645,6 +668,10 @@ int plan(int es, int rank, const int *dims, const int *o2n, Planned &pl, int cus
       break;
     }
   }
+  // the kernel writes one vector to consecutive LDS slots, so a vector must
+  // not straddle a padding period (every candidate period above is a
+  // multiple of the width already; this keeps that invariant explicit)
+  while (vec > 1 && best % (uint32_t)vec != 0) vec /= 2;
   pl.vec = vec;
   P.tv_in0 = to_dev((uint32_t)(tp.t[tin[0]] / vec));
   P.tv_out0 = to_dev((uint32_t)(tp.t[tout[0]] / vec));
@@ -655,9 +682,29 @@ int plan(int es, int rank, const int *dims, const int *o2n, Planned &pl, int cus
   pl.rag = P.nrag > 0;
   const size_t slots = P.vol_tile + (best ? P.vol_tile / best + 1 : 0);
   pl.lds_bytes = slots * (size_t)es;
-  const int per_cu = std::max(1, std::min(8, (int)(160 * 1024 / std::max<size_t>(pl.lds_bytes, 1))));
+  // resident workgroups per CU: measured on the transpose workload, four (16
+  // waves) move more bytes per second than eight (the DRAM sees fewer
+  // concurrent streams); LDS may allow fewer
+  const int per_cu = std::max(1, std::min(kTiledPerCU, (int)(160 * 1024 / std::max<size_t>(pl.lds_bytes, 1))));
+  pl.cus = cus;
   pl.grid = (int)std::min<uint64_t>(ntiles, (uint64_t)cus * per_cu);
   return 0;
+}
+
+// Workgroups of one kernel instantiation that a CU holds at once as far as its
+// registers allow (LDS is accounted for by the planner); the persistent grid
+// is capped at this so that every workgroup is resident from the start.
+template <typename W, int VEC, int EPT, bool CONJ, typename OffT, bool RAG>
+int resident_per_cu()
+{
+  static const int n = [] {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_permute_tiled<W, VEC, EPT, CONJ, OffT, RAG>, kNT, 0) !=
+            hipSuccess || b <= 0)
+      b = 1;
+    return b;
+  }();
+  return n;
 }
 
 template <typename W, int VEC, bool CONJ, typename OffT, bool RAG>
@@ -665,12 +712,15 @@ int launch_tiled_v(const Planned &pl, const void *in, void *out, hipStream_t s)
 {
   const W *a = static_cast<const W *>(in);
   W *b = static_cast<W *>(out);
-  dim3 g(pl.grid), blk(kNT);
+  const dim3 blk(kNT);
   switch (pl.ept) {
 #define XT_CASE(E)                                                                                             \
-  case E:                                                                                                      \
+  case E: {                                                                                                    \
+    const int cap = pl.cus * resident_per_cu<W, VEC, E, CONJ, OffT, RAG>();                                    \
+    const dim3 g(std::min(pl.grid, cap));                                                                      \
     hipLaunchKernelGGL((k_permute_tiled<W, VEC, E, CONJ, OffT, RAG>), g, blk, pl.lds_bytes, s, a, b, pl.tp);   \
-    break;
+    break;                                                                                                     \
+  }
     XT_CASE(1) XT_CASE(2) XT_CASE(4) XT_CASE(8) XT_CASE(16)
 #undef XT_CASE
   default: return TALSH_FAILURE;
