@@ -10,10 +10,11 @@
 //    and in order in the output, then pick an LDS tile = the union of an
 //    input-minor dimension group and an output-minor dimension group, grown
 //    until both contiguous runs reach >= 256 bytes (the last dimension of each
-//    group may be split; ragged splits are masked);
+//    group may be split, preferably so that the run is a whole number of
+//    128-byte lines; ragged splits are masked), then grown to >= 8192 elements;
 //  * LDS padding is chosen on the host by simulating the bank conflicts of the
 //    store phase for each candidate padding period;
-//  * persistent kernel (four resident workgroups per CU): each thread decodes
+//  * persistent kernel (two resident 512-thread workgroups per CU): each thread decodes
 //    the intra-tile offsets of its vectors once (input offset, output offset,
 //    LDS read base) and keeps them in registers, so the per-tile work is one
 //    base-offset decode per workgroup plus one coalesced load and one
@@ -30,11 +31,15 @@
 namespace xt {
 namespace {
 
-constexpr int kNT = 256;           // threads per workgroup
+constexpr int kNT = 256;           // threads per workgroup (copy and scatter kernels)
+constexpr int kTNT = 512;          // threads per workgroup of the tiled kernel
 constexpr int kMaxTileDims = 8;    // dimensions inside one tile
 constexpr int kMaxGridDims = 16;   // fused rank handled by the tiled path
-constexpr int kMaxEPT = 16;        // tile elements per thread
-constexpr int kTiledPerCU = 4;     // persistent workgroups per CU
+constexpr int kMaxEPT = 8;         // tile vectors per thread (more cost occupancy)
+constexpr int kTiledPerCU = 2;     // persistent workgroups per CU (16 waves)
+constexpr int kTileMaxBytes = 65536;  // LDS tile (before padding)
+constexpr long kTileMinVol = 8192;    // tiles are grown to at least this many elements
+constexpr int kLine = 128;            // cache line bytes
 
 struct DevDiv { uint32_t d, m, s; };
 
@@ -129,7 +134,7 @@ __device__ __forceinline__ void st_stream(P *p, const P &v)
 // being held for every element, which keeps the common instantiations at
 // 30-60 VGPRs, i.e. at full occupancy.
 template <typename W, int VEC, int EPT, bool CONJ, typename OffT, bool RAG>
-__global__ __launch_bounds__(kNT) void k_permute_tiled(const W *__restrict__ in, W *__restrict__ out,
+__global__ __launch_bounds__(kTNT) void k_permute_tiled(const W *__restrict__ in, W *__restrict__ out,
                                                         const TileParams p)
 {
   typedef Pack<W, VEC> P;
@@ -143,7 +148,7 @@ __global__ __launch_bounds__(kNT) void k_permute_tiled(const W *__restrict__ in,
   // one-time decode of this thread's tile vectors (kept in registers)
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
-    const uint32_t v = tid + (uint32_t)k * kNT;
+    const uint32_t v = tid + (uint32_t)k * kTNT;
     uint32_t r = v, cin[2] = {0, 0}, cout[2] = {0, 0};
     OffT io = 0, oo = 0;
     uint32_t ls = 0;
@@ -209,7 +214,7 @@ __global__ __launch_bounds__(kNT) void k_permute_tiled(const W *__restrict__ in,
   auto load = [&](const TileBase &s) {
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
-      const uint32_t v = tid + (uint32_t)k * kNT;
+      const uint32_t v = tid + (uint32_t)k * kTNT;
       bool ok = v < nvec;
       if (RAG && !s.full) ok = ok && (rin[k] & 0xFFFFu) < s.rem0 && (rin[k] >> 16) < s.rem1;
       if (ok) val[k] = ld_stream(reinterpret_cast<const P *>(in + (s.bin + in_off[k])));
@@ -220,7 +225,7 @@ __global__ __launch_bounds__(kNT) void k_permute_tiled(const W *__restrict__ in,
   auto to_lds = [&](const TileBase &s) {
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
-      uint32_t v = tid + (uint32_t)k * kNT;
+      uint32_t v = tid + (uint32_t)k * kTNT;
       bool ok = v < nvec;
       if (RAG && !s.full) ok = ok && (rin[k] & 0xFFFFu) < s.rem0 && (rin[k] >> 16) < s.rem1;
       if (ok) {
@@ -235,7 +240,7 @@ __global__ __launch_bounds__(kNT) void k_permute_tiled(const W *__restrict__ in,
   auto from_lds = [&](const TileBase &s) {
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
-      const uint32_t f = tid + (uint32_t)k * kNT;
+      const uint32_t f = tid + (uint32_t)k * kTNT;
       bool ok = f < nvec;
       if (RAG && !s.full) ok = ok && (rout[k] & 0xFFFFu) < s.rem0 && (rout[k] >> 16) < s.rem1;
       if (ok) {
@@ -370,9 +375,16 @@ Fused fuse_dims(int rank, const int *dims, const int *o2n)
   return f;
 }
 
-// Smallest divisor of ext in [need, 2*need], else need itself.
-int pick_split(int ext, int need)
+// Smallest divisor of ext in [need, 2*need], else need itself. With
+// unit_bytes > 0 (the bytes of the contiguous run the split extends), the
+// smallest divisor in [need, 4*need] that makes the run a whole number of
+// cache lines is preferred: runs that end mid-line leave lines split between
+// tiles.
+int pick_split(int ext, int need, long unit_bytes = 0)
 {
+  if (unit_bytes > 0)
+    for (int t = need; t <= std::min(ext, 4 * need); ++t)
+      if (ext % t == 0 && (unit_bytes * t) % kLine == 0) return t;
   for (int t = need; t <= std::min(ext, 2 * need); ++t)
     if (ext % t == 0) return t;
   return std::min(ext, need);
@@ -384,7 +396,7 @@ struct TilePlan {
   int in_run = 0, out_run = 0;
 };
 
-TilePlan choose_tile(const Fused &f, int target_in, int target_out)
+TilePlan choose_tile(const Fused &f, int target_in, int target_out, int align_es)
 {
   const int r = f.rank;
   std::vector<int> n2o(r);
@@ -399,7 +411,7 @@ TilePlan choose_tile(const Fused &f, int target_in, int target_out)
       run *= f.ext[i];
       if (run >= target_in) break;
     } else {
-      tp.t[i] = pick_split(f.ext[i], (int)need);
+      tp.t[i] = pick_split(f.ext[i], (int)need, run * align_es);
       run *= tp.t[i];
       break;
     }
@@ -409,8 +421,9 @@ TilePlan choose_tile(const Fused &f, int target_in, int target_out)
   for (int q = 0; q < r; ++q) {
     const int i = n2o[q];
     const long need = (target_out + run - 1) / run;
-    if (tp.t[i] > 0 && tp.t[i] < f.ext[i] && tp.t[i] < need) tp.t[i] = pick_split(f.ext[i], (int)std::min<long>(need, f.ext[i]));
-    if (tp.t[i] == 0) tp.t[i] = need >= f.ext[i] ? f.ext[i] : pick_split(f.ext[i], (int)need);
+    if (tp.t[i] > 0 && tp.t[i] < f.ext[i] && tp.t[i] < need)
+      tp.t[i] = pick_split(f.ext[i], (int)std::min<long>(need, f.ext[i]), run * align_es);
+    if (tp.t[i] == 0) tp.t[i] = need >= f.ext[i] ? f.ext[i] : pick_split(f.ext[i], (int)need, run * align_es);
     run *= tp.t[i];
     if (tp.t[i] < f.ext[i] || run >= target_out) break;
   }
@@ -534,6 +547,24 @@ int num_cus_current()
   return cus;
 }
 
+// Vector width the kernel can use for a tile: both minor dimensions (the first
+// tile dimension in input and in output order) and their tile extents must be
+// multiples of it, so that every vector is aligned and whole.
+int tile_vec(const Fused &f, const TilePlan &tp, int es)
+{
+  int i0 = -1, o0 = -1;
+  for (int i = 0; i < f.rank && i0 < 0; ++i)
+    if (tp.t[i] > 0) i0 = i;
+  std::vector<int> n2o(f.rank);
+  for (int i = 0; i < f.rank; ++i) n2o[f.o2n[i]] = i;
+  for (int q = 0; q < f.rank && o0 < 0; ++q)
+    if (tp.t[n2o[q]] > 0) o0 = n2o[q];
+  if (i0 < 0 || o0 < 0) return 1;
+  for (int cand = 16 / es; cand >= 2; cand /= 2)
+    if (f.ext[i0] % cand == 0 && tp.t[i0] % cand == 0 && f.ext[o0] % cand == 0 && tp.t[o0] % cand == 0) return cand;
+  return 1;
+}
+
 int plan(int es, int rank, const int *dims, const int *o2n, Planned &pl, int cus, bool choose_pad = true)
 {
   pl.f = fuse_dims(rank, dims, o2n);
@@ -544,24 +575,30 @@ int plan(int es, int rank, const int *dims, const int *o2n, Planned &pl, int cus
   if (f.rank > kMaxGridDims) { pl.path = 2; return 0; }
   uint64_t vol = 1;
   for (int e : f.ext) vol *= (uint64_t)e;
-  const int max_tile_bytes = 32768;
-  int target = std::max(8, 256 / es);
+  // two attempts: first with splits that keep runs whole cache lines, then
+  // without, if the aligned tile would need more vectors per thread than the
+  // kernel is instantiated for
   TilePlan tp;
-  while (true) {
-    tp = choose_tile(f, target, target);
-    int ntd = 0;
-    for (int x : tp.t) ntd += x > 0;
-    if ((tp.vol * es <= max_tile_bytes && tp.vol <= (long)kNT * kMaxEPT && ntd <= kMaxTileDims) || target <= 1) break;
-    target = std::max(1, target / 2);
-  }
-  {
-    const long max_vol = std::min<long>((long)kNT * kMaxEPT, max_tile_bytes / es);
-    const long min_vol = std::min<long>(2048, max_vol);
+  bool ok = false;
+  for (int attempt = 0; attempt < 2 && !ok; ++attempt) {
+    const int align_es = attempt == 0 ? es : 0;
+    // the second attempt bounds the tile by one element per vector, so that
+    // it always fits the kernel's vector slots
+    const long max_vol = std::min<long>((long)kTNT * kMaxEPT * (attempt == 0 ? 16 / es : 1), kTileMaxBytes / es);
+    int target = std::max(8, 256 / es);
+    while (true) {
+      tp = choose_tile(f, target, target, align_es);
+      int ntd = 0;
+      for (int x : tp.t) ntd += x > 0;
+      if ((tp.vol <= max_vol && ntd <= kMaxTileDims) || target <= 1) break;
+      target = std::max(1, target / 2);
+    }
     TilePlan g = tp;
-    grow_tile(f, g, min_vol, max_vol);
+    grow_tile(f, g, std::min(kTileMinVol, max_vol), max_vol);
     int gd = 0;
     for (int x : g.t) gd += x > 0;
     if (gd <= kMaxTileDims) tp = g;
+    ok = tp.vol <= max_vol && (tp.vol + (long)kTNT * tile_vec(f, tp, es) - 1) / ((long)kTNT * tile_vec(f, tp, es)) <= kMaxEPT;
   }
   int ntd = 0, nrag = 0;
   for (int i = 0; i < f.rank; ++i) {
@@ -570,8 +607,7 @@ int plan(int es, int rank, const int *dims, const int *o2n, Planned &pl, int cus
   }
   uint64_t ntiles = 1;
   for (int i = 0; i < f.rank; ++i) ntiles *= (uint64_t)((f.ext[i] + std::max(tp.t[i], 1) - 1) / std::max(tp.t[i], 1));
-  if (tp.vol * es > max_tile_bytes || ntd > kMaxTileDims || nrag > 2 || ntiles >= (1ull << 31) ||
-      tp.vol > (long)kNT * kMaxEPT) {
+  if (!ok || ntd > kMaxTileDims || nrag > 2 || ntiles >= (1ull << 31)) {
     pl.path = 2;
     return 0;
   }
@@ -637,7 +673,7 @@ int plan(int es, int rank, const int *dims, const int *o2n, Planned &pl, int cus
   P.ntiles = (uint32_t)ntiles;
   // LDS padding by simulated store-phase conflicts (first 256 store slots)
   std::vector<uint32_t> st;
-  for (uint32_t e = 0; e < std::min<uint32_t>(P.vol_tile, kNT); ++e) {
+  for (uint32_t e = 0; e < std::min<uint32_t>(P.vol_tile, kTNT); ++e) {
     uint32_t r = e, l = 0;
     for (int j = 0; j < P.nt; ++j) {
       const uint32_t d = P.t_out[j].d;
@@ -660,14 +696,7 @@ int plan(int es, int rank, const int *dims, const int *o2n, Planned &pl, int cus
   P.pad = to_dev(best);
   // vector width: both minor dimensions (input dim 0, output dim 0) and their
   // tile extents must be multiples of it so every vector is aligned and whole
-  int vec = 1;
-  for (int cand = 16 / es; cand >= 2; cand /= 2) {
-    const int i0 = tin[0], o0 = tout[0];
-    if (f.ext[i0] % cand == 0 && tp.t[i0] % cand == 0 && f.ext[o0] % cand == 0 && tp.t[o0] % cand == 0) {
-      vec = cand;
-      break;
-    }
-  }
+  int vec = tile_vec(f, tp, es);
   // the kernel writes one vector to consecutive LDS slots, so a vector must
   // not straddle a padding period (every candidate period above is a
   // multiple of the width already; this keeps that invariant explicit)
@@ -676,15 +705,15 @@ int plan(int es, int rank, const int *dims, const int *o2n, Planned &pl, int cus
   P.tv_in0 = to_dev((uint32_t)(tp.t[tin[0]] / vec));
   P.tv_out0 = to_dev((uint32_t)(tp.t[tout[0]] / vec));
   int ept = 1;
-  while ((long)ept * kNT * vec < tp.vol) ept *= 2;
+  while ((long)ept * kTNT * vec < tp.vol) ept *= 2;
   pl.ept = ept;
   pl.wide = vol >= (1ull << 32);
   pl.rag = P.nrag > 0;
   const size_t slots = P.vol_tile + (best ? P.vol_tile / best + 1 : 0);
   pl.lds_bytes = slots * (size_t)es;
-  // resident workgroups per CU: measured on the transpose workload, four (16
-  // waves) move more bytes per second than eight (the DRAM sees fewer
-  // concurrent streams); LDS may allow fewer
+  // resident workgroups per CU: measured on the transpose workload, 16 waves
+  // per CU move more bytes per second than 32 (the DRAM sees fewer concurrent
+  // streams); LDS may allow fewer
   const int per_cu = std::max(1, std::min(kTiledPerCU, (int)(160 * 1024 / std::max<size_t>(pl.lds_bytes, 1))));
   pl.cus = cus;
   pl.grid = (int)std::min<uint64_t>(ntiles, (uint64_t)cus * per_cu);
@@ -699,7 +728,7 @@ int resident_per_cu()
 {
   static const int n = [] {
     int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_permute_tiled<W, VEC, EPT, CONJ, OffT, RAG>, kNT, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_permute_tiled<W, VEC, EPT, CONJ, OffT, RAG>, kTNT, 0) !=
             hipSuccess || b <= 0)
       b = 1;
     return b;
@@ -712,7 +741,7 @@ int launch_tiled_v(const Planned &pl, const void *in, void *out, hipStream_t s)
 {
   const W *a = static_cast<const W *>(in);
   W *b = static_cast<W *>(out);
-  const dim3 blk(kNT);
+  const dim3 blk(kTNT);
   switch (pl.ept) {
 #define XT_CASE(E)                                                                                             \
   case E: {                                                                                                    \
@@ -721,7 +750,7 @@ int launch_tiled_v(const Planned &pl, const void *in, void *out, hipStream_t s)
     hipLaunchKernelGGL((k_permute_tiled<W, VEC, E, CONJ, OffT, RAG>), g, blk, pl.lds_bytes, s, a, b, pl.tp);   \
     break;                                                                                                     \
   }
-    XT_CASE(1) XT_CASE(2) XT_CASE(4) XT_CASE(8) XT_CASE(16)
+    XT_CASE(1) XT_CASE(2) XT_CASE(4) XT_CASE(8)
 #undef XT_CASE
   default: return TALSH_FAILURE;
   }
@@ -860,10 +889,12 @@ int launch_permute(const void *in, void *out, int kind, int rank, const int *dim
       pl.tp.tv_out0 = to_dev(pl.tp.t_out[0].d);
       pl.vec = 1;
       pl.ept = 1;
-      while ((long)pl.ept * kNT < (long)pl.tp.vol_tile) pl.ept *= 2;
+      while ((long)pl.ept * kTNT < (long)pl.tp.vol_tile) pl.ept *= 2;
+      if (pl.ept > kMaxEPT) pl.path = 2;  // one element per access: too many per thread for the tiled kernel
     }
     // the persistent grid and per-thread vector slots must cover the tile
-    if ((long)pl.ept * kNT * pl.vec < (long)pl.tp.vol_tile || pl.grid <= 0 || pl.ept > kMaxEPT) return TALSH_FAILURE;
+    if (pl.path == 1 && ((long)pl.ept * kTNT * pl.vec < (long)pl.tp.vol_tile || pl.grid <= 0 || pl.ept > kMaxEPT))
+      return TALSH_FAILURE;
   }
   const bool cj = conj && kind_complex(kind);
   hipEvent_t pr[2];

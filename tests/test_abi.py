@@ -1,7 +1,8 @@
 """The drop-in boundary: the engine library exports every entry point its
 headers declare, with the reference's struct layouts and constant values, and
 its host-side index logic (pattern parser, matricization planner) matches the
-reference's golden fixtures. CPU only."""
+reference's golden fixtures, and the transpose planner's tiles fit the kernel.
+CPU only."""
 import ctypes as C
 import gzip
 import json
@@ -114,6 +115,48 @@ def test_permutation_planner_matches_reference_fixtures(lib):
         assert list(l[1:row["lrank"] + 1]) == row["lprm"][1:], row
         assert list(r[1:row["rrank"] + 1]) == row["rprm"][1:], row
         assert list(d[1:len(row["dprm"])]) == row["dprm"][1:], row
+
+
+def test_transpose_plans_fit_the_kernel(lib):
+    """Every tiled transpose plan fits the kernel it is launched with: the LDS
+    tile is at most 64 KiB before padding, a 512-thread workgroup covers it
+    with at most 8 vectors per thread, and the vector width divides 16 bytes.
+    On the rank-6 extent-24 and rank-4 extent-128 workloads every plan is
+    tiled, uses 16-byte vectors, and both contiguous runs are whole 128-byte
+    lines. Random shapes of ranks 2-8 and all four data kinds, CPU only."""
+    import math
+    import random
+    es_of = {T.R4: 4, T.R8: 8, T.C4: 8, T.C8: 16}
+    rng = random.Random(11)
+
+    def plan(kind, dims, perm):
+        info = T.ints([0] * 7)
+        assert lib.talshAmdPermutePlan(kind, len(dims), T.ints(dims), T.ints(perm), info) == 0
+        return list(info)
+
+    for _ in range(600):
+        kind = rng.choice(list(es_of))
+        es = es_of[kind]
+        rank = rng.randint(2, 8)
+        dims = [rng.choice((1, 2, 3, 5, 7, 8, 16, 24, 31, 64, 100, 128, 257)) for _ in range(rank)]
+        while math.prod(dims) > 1 << 28:
+            dims[dims.index(max(dims))] //= 2
+        perm = list(range(rank))
+        rng.shuffle(perm)
+        fused, path, tile, in_run, out_run, grid, vec = plan(kind, dims, perm)
+        if path != 1:
+            continue
+        assert tile * es <= 65536, (dims, perm, tile)
+        assert -(-tile // (512 * vec)) <= 8, (dims, perm, tile, vec)
+        assert vec * es <= 16 and grid >= 1
+    import bench
+    for kind in (T.R4, T.R8):
+        es = es_of[kind]
+        for dims, perms in (([24] * 6, bench.perms_rank6()), ([128] * 4, bench.perms_rank4())):
+            for p in perms:
+                fused, path, tile, in_run, out_run, grid, vec = plan(kind, dims, p)
+                assert path == 1 and vec == 16 // es, (kind, p)
+                assert (in_run * es) % 128 == 0 and (out_run * es) % 128 == 0, (kind, p, in_run, out_run)
 
 
 @pytest.fixture(scope="module")
