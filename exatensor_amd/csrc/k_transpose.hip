@@ -311,28 +311,24 @@ __global__ __launch_bounds__(kNT) void k_copy(const T *__restrict__ in, T *__res
   for (uint64_t i = (uint64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += stride) out[i] = conj_if<T, CONJ>(in[i]);
 }
 
-// Identity permutation of a 16-byte aligned body: 16-byte streaming accesses,
-// four in flight per thread per iteration.
+// Identity permutation of a 16-byte aligned body: one 16-byte streaming
+// access per thread in one-wave workgroups, one workgroup per KiB. The
+// dispatcher hands the workgroups out in index order, so the lines in flight
+// at any moment form a compact window that sweeps the body; on this part
+// that moves more bytes per second than any persistent grid-stride form
+// (measured: 6.4-6.7 TB/s against 5.5-5.8 TB/s).
+constexpr int kCopyNT = 64;
 template <typename W, bool CONJ>
-__global__ __launch_bounds__(kNT) void k_copy16(const nt4 *__restrict__ in, nt4 *__restrict__ out, uint64_t n16)
+__global__ __launch_bounds__(kCopyNT) void k_copy16(const nt4 *__restrict__ in, nt4 *__restrict__ out, uint64_t n16)
 {
-  constexpr int U = 4;
-  const uint64_t stride = (uint64_t)gridDim.x * kNT * U;
-  for (uint64_t base = (uint64_t)blockIdx.x * kNT * U + threadIdx.x; base < n16; base += stride) {
-    nt4 v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (base + (uint64_t)u * kNT < n16) v[u] = __builtin_nontemporal_load(in + base + (uint64_t)u * kNT);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (base + (uint64_t)u * kNT >= n16) continue;
-      if (CONJ) {  // complex words: flip the sign bit of each imaginary part
-        if (sizeof(W) == 8) { v[u].y ^= 0x80000000u; v[u].w ^= 0x80000000u; }
-        else v[u].w ^= 0x80000000u;
-      }
-      __builtin_nontemporal_store(v[u], out + base + (uint64_t)u * kNT);
-    }
+  const uint64_t i = (uint64_t)blockIdx.x * kCopyNT + threadIdx.x;
+  if (i >= n16) return;
+  nt4 v = __builtin_nontemporal_load(in + i);
+  if (CONJ) {  // complex words: flip the sign bit of each imaginary part
+    if (sizeof(W) == 8) { v.y ^= 0x80000000u; v.w ^= 0x80000000u; }
+    else v.w ^= 0x80000000u;
   }
+  __builtin_nontemporal_store(v, out + i);
 }
 
 // ---------------------------------------------------------------------------
@@ -785,8 +781,11 @@ int launch_copy_t(const void *in, void *out, uint64_t n, hipStream_t s, int cus)
   const uint64_t bytes = n * sizeof(W);
   if (bytes % 16 == 0 && ((uintptr_t)in % 16) == 0 && ((uintptr_t)out % 16) == 0) {
     const uint64_t n16 = bytes / 16;
-    const uint64_t blocks = std::min<uint64_t>((n16 + kNT * 4 - 1) / (kNT * 4), (uint64_t)cus * 8);
-    hipLaunchKernelGGL((k_copy16<W, CONJ>), dim3((unsigned)std::max<uint64_t>(blocks, 1)), dim3(kNT), 0, s,
+    const uint64_t blocks = (n16 + kCopyNT - 1) / kCopyNT;
+    // one launch covers up to 2^31 - 1 workgroups of 1 KiB (2 TiB): always
+    // enough for a body that fits in device memory
+    if (blocks >= (1ull << 31)) return TALSH_INVALID_ARGS;
+    hipLaunchKernelGGL((k_copy16<W, CONJ>), dim3((unsigned)blocks), dim3(kCopyNT), 0, s,
                        static_cast<const nt4 *>(in), static_cast<nt4 *>(out), n16);
     return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
   }
