@@ -28,17 +28,25 @@ namespace {
 
 Task *task_of(const talsh_task_t *t) { return t ? static_cast<Task *>(t->task_p) : nullptr; }
 
-int new_task(talsh_task_t *tt, Device *d, int kind, unsigned coh, int nargs, hipStream_t prefer = nullptr)
+void destroy_task_events(Task *k);
+
+int new_task(talsh_task_t *tt, Device *d, int kind, unsigned coh, int nargs, hipStream_t prefer, bool timed)
 {
   Task *k = new Task();
+  k->timed = timed;
   k->dev_flat = d->flat_id;
   k->stream = d->streams[d->next_stream.fetch_add(1) % kStreamsPerDevice];
   for (hipStream_t st : d->streams)
     if (prefer && st == prefer) k->stream = prefer;
   bool ok = true;
-  for (hipEvent_t *e : {&k->ev_start, &k->ev_comput, &k->ev_output, &k->ev_finish, &k->ev_mm0, &k->ev_mm1})
-    ok = ok && (*e = event_get(true)) != nullptr;
+  if (timed) {
+    for (hipEvent_t *e : {&k->ev_start, &k->ev_comput, &k->ev_output, &k->ev_finish, &k->ev_mm0, &k->ev_mm1})
+      ok = ok && (*e = event_get(true)) != nullptr;
+  } else {
+    ok = (k->ev_finish = event_get(false)) != nullptr;  // an untimed task has only its completion event
+  }
   if (!ok) {
+    destroy_task_events(k);
     delete k;
     return TALSH_FAILURE;
   }
@@ -57,8 +65,13 @@ int new_task(talsh_task_t *tt, Device *d, int kind, unsigned coh, int nargs, hip
 
 void destroy_task_events(Task *k)
 {
-  for (hipEvent_t e : {k->ev_start, k->ev_comput, k->ev_output, k->ev_finish, k->ev_mm0, k->ev_mm1})
-    event_put(e, true);
+  if (k->timed) {
+    for (hipEvent_t e : {k->ev_start, k->ev_comput, k->ev_output, k->ev_finish, k->ev_mm0, k->ev_mm1})
+      event_put(e, true);
+  } else {
+    event_put(k->ev_finish, false);
+  }
+  k->ev_start = k->ev_comput = k->ev_output = k->ev_finish = k->ev_mm0 = k->ev_mm1 = nullptr;
 }
 
 // Prepared operand of an operation.
@@ -254,7 +267,7 @@ int task_finalize(talsh_task_t *tt, int status)
   k->finalized = true;
   tt->task_error = status == TALSH_TASK_COMPLETED ? 0 : 1;
   float ms = 0;
-  if (status == TALSH_TASK_COMPLETED && hipEventElapsedTime(&ms, k->ev_start, k->ev_finish) == hipSuccess)
+  if (status == TALSH_TASK_COMPLETED && k->timed && hipEventElapsedTime(&ms, k->ev_start, k->ev_finish) == hipSuccess)
     tt->exec_time = ms * 1e-3;
   return 0;
 }
@@ -301,7 +314,9 @@ int run_task(talsh_task_t *user, Device *d, unsigned coh, std::vector<Arg> &args
   if (!user)
     for (const Arg &a : args)
       if (a.output && (prefer = image_last_stream(a.exec)) != nullptr) break;
-  e = new_task(tt, d, kind, coh, nargs, prefer);
+  // the task of a blocking call is never queried for its timings, so it records
+  // only its completion event (every recorded event adds latency to the call)
+  e = new_task(tt, d, kind, coh, nargs, prefer, user != nullptr);
   if (e) {
     temps.rollback();
     for (Arg &a : args) unprepare(a);
@@ -314,11 +329,11 @@ int run_task(talsh_task_t *user, Device *d, unsigned coh, std::vector<Arg> &args
     tt->tens_args[a].source_image = 0;
   }
   d->stats.tasks_submitted++;
-  (void)hipEventRecord(k->ev_start, k->stream);
+  if (k->timed) (void)hipEventRecord(k->ev_start, k->stream);
   enqueue_inputs(k, args);
-  (void)hipEventRecord(k->ev_comput, k->stream);
+  if (k->timed) (void)hipEventRecord(k->ev_comput, k->stream);
   e = body.enqueue(k, d);
-  (void)hipEventRecord(k->ev_output, k->stream);
+  if (k->timed) (void)hipEventRecord(k->ev_output, k->stream);
   enqueue_outputs(k, args);
   (void)hipEventRecord(k->ev_finish, k->stream);
   {
@@ -1043,12 +1058,12 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
       g.a_kfast = P.a_kfast;
       g.b_kfast = P.b_kfast;
       g.c = dc;
-      (void)hipEventRecord(k->ev_mm0, s);
+      if (k->timed) (void)hipEventRecord(k->ev_mm0, s);
       prof_begin(s, kKGemm, pr);
       int e = launch_gemm_ws(g, ws, s);
       prof_end(s, kKGemm, pr);
-      (void)hipEventRecord(k->ev_mm1, s);
-      k->has_mm = true;
+      if (k->timed) (void)hipEventRecord(k->ev_mm1, s);
+      k->has_mm = k->timed;
       if (e) return e;
       if (dt) {
         e = launch_permute(dt, D->exec, kind, (int)P.dmat_dims.size(), P.dmat_dims.data(), P.d_out_o2n.data(), false,

@@ -68,6 +68,7 @@ struct Task {
   hipEvent_t ev_start = nullptr, ev_comput = nullptr, ev_output = nullptr, ev_finish = nullptr, ev_mm0 = nullptr,
              ev_mm1 = nullptr;
   bool has_mm = false;
+  bool timed = true;  // start/input/output/GEMM events recorded (a blocking call's task records only its finish)
   int status = TALSH_TASK_EMPTY;
   bool finalized = false;
   std::vector<PendingFree> frees;      // temporaries released at completion
