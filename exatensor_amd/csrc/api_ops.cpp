@@ -983,6 +983,7 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
       if (!e) {
         GemmArgs g{};
         g.kind = D->kind; g.r4mode = t.d->r4mode; g.m = P.m; g.n = P.n; g.k = P.k;
+        g.tile = tile;  // the split-K workspace depends on the tile shape
         wsb = gemm_workspace_bytes(g);
         if (wsb) e = t.alloc(wsb, &ws);
       }
@@ -1557,12 +1558,15 @@ int talshAmdGemm(int data_kind, int fast_math, size_t m, size_t n, size_t k, con
   g.m = m; g.n = n; g.k = k; g.a = a; g.b = b; g.c = c;
   g.a_kfast = a_kfast; g.b_kfast = b_kfast; g.beta_zero = beta_zero;
   g.alpha_re = alpha_re; g.alpha_im = alpha_im; g.tile = tile;
-  if (gemm_workspace_bytes(g) > workspace_bytes) return TALSH_INVALID_ARGS;
+  const size_t need = gemm_workspace_bytes(g);
+  // the three-way split cannot run without its workspace; a real product
+  // without room for split-K partials runs unsplit
+  if (g.r4mode == R4Mode::kBF16x3 && data_kind == R4 && need > workspace_bytes) return TALSH_INVALID_ARGS;
   (void)hipSetDevice(d->ordinal);
   hipStream_t s = d->streams[0];
   hipEvent_t pr[2];
   prof_begin(s, kKGemm, pr);
-  const int e = launch_gemm_ws(g, workspace, s);
+  const int e = launch_gemm_ws(g, need <= workspace_bytes ? workspace : nullptr, s);
   prof_end(s, kKGemm, pr);
   if (e) return e;
   return XT_HIP_OK(hipStreamSynchronize(s)) ? 0 : TALSH_FAILURE;
@@ -1574,7 +1578,12 @@ size_t talshAmdGemmWorkspace(int data_kind, int fast_math, size_t m, size_t n, s
   g.kind = data_kind;
   g.r4mode = fast_math ? R4Mode::kBF16x3 : R4Mode::kNativeF32;
   g.m = m; g.n = n; g.k = k;
-  return gemm_workspace_bytes(g);
+  size_t bytes = 0;
+  for (int tile = -1; tile < 3; ++tile) {  // enough for every tile choice
+    g.tile = tile;
+    bytes = std::max(bytes, gemm_workspace_bytes(g));
+  }
+  return bytes;
 }
 
 int talshAmdDeviceSynchronize(int dev_num)

@@ -52,6 +52,11 @@ struct GemmParams {
   int tiles_m, tiles_n;
   int xcd_remap;             // 1: blockIdx -> tile remap that keeps neighbours on one XCD
   int vec_a, vec_b;          // 16-byte aligned rows of k-fast operands (vector loads allowed)
+  // split K (real kinds): workgroup row blockIdx.y takes k in [y * kchunk,
+  // (y + 1) * kchunk) and writes its raw m x n partial to part + y * m * n;
+  // part == nullptr: one pass over all of K, epilogue into C
+  void *part;
+  int64_t kchunk;
 };
 
 // Maps a workgroup to an output tile: first gather the workgroups that the
@@ -109,6 +114,9 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
   const T *A = static_cast<const T *>(p.a);
   const T *B = static_cast<const T *>(p.b);
   const bool interior_mn = (m0 + BM <= p.m) && (n0 + BN <= p.n);
+  const bool split = !CPLX && p.part != nullptr;
+  const int64_t kbeg = split ? (int64_t)blockIdx.y * p.kchunk : 0;
+  const int64_t kend = split ? min(p.k, kbeg + p.kchunk) : p.k;
 
   // staging coordinates: k-fast operands: a thread covers LA consecutive k of
   // one row; row-fast operands: LA consecutive rows of one k.
@@ -120,7 +128,7 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
 
   T ra[NP][LA], rb[NP][LB];
   auto load_a = [&](int64_t k0) {
-    const bool vec = !CPLX && AK && interior_mn && (k0 + BK <= p.k) && p.vec_a;
+    const bool vec = !CPLX && AK && interior_mn && (k0 + BK <= kend) && p.vec_a;
     if (vec) {
       const T *src = A + (m0 + a_r) * p.lda + k0 + a_k;
 #pragma unroll
@@ -130,7 +138,7 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
 #pragma unroll
     for (int i = 0; i < LA; ++i) {
       const int64_t r = m0 + a_r + (AK ? 0 : i), kk = k0 + a_k + (AK ? i : 0);
-      const bool ok = r < p.m && kk < p.k;
+      const bool ok = r < p.m && kk < kend;
       const int64_t off = AK ? (r * p.lda + kk) : (kk * p.lda + r);
       if (CPLX) {
         ra[0][i] = ok ? A[2 * off] : T(0);
@@ -142,7 +150,7 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
     }
   };
   auto load_b = [&](int64_t k0) {
-    const bool vec = !CPLX && BKF && interior_mn && (k0 + BK <= p.k) && p.vec_b;
+    const bool vec = !CPLX && BKF && interior_mn && (k0 + BK <= kend) && p.vec_b;
     if (vec) {
       const T *src = B + (n0 + b_r) * p.ldb + k0 + b_k;
 #pragma unroll
@@ -152,7 +160,7 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
 #pragma unroll
     for (int i = 0; i < LB; ++i) {
       const int64_t r = n0 + b_r + (BKF ? 0 : i), kk = k0 + b_k + (BKF ? i : 0);
-      const bool ok = r < p.n && kk < p.k;
+      const bool ok = r < p.n && kk < kend;
       const int64_t off = BKF ? (r * p.ldb + kk) : (kk * p.ldb + r);
       if (CPLX) {
         rb[0][i] = ok ? B[2 * off] : T(0);
@@ -193,17 +201,17 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
 #pragma unroll
       for (int j = 0; j < FN; ++j) acc[q][i][j] = AccT{0, 0, 0, 0};
 
-  const int64_t nk = (p.k + BK - 1) / BK;
-  load_a(0);
-  load_b(0);
+  const int64_t nk = (kend - kbeg + BK - 1) / BK;
+  load_a(kbeg);
+  load_b(kbeg);
   store(0);
   __syncthreads();
   const int fr = lane & 15, fg = (lane >> 4) * 4;
   for (int64_t kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nk) {
-      load_a((kt + 1) * BK);
-      load_b((kt + 1) * BK);
+      load_a(kbeg + (kt + 1) * BK);
+      load_b(kbeg + (kt + 1) * BK);
     }
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 16) {
@@ -242,8 +250,9 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
     if (kt + 1 < nk) store(cur ^ 1);
     __syncthreads();
   }
-  // epilogue: C = alpha * acc + beta * C (beta 0 or 1)
+  // epilogue: C = alpha * acc + beta * C (beta 0 or 1), or the raw partial
   T *C = static_cast<T *>(p.c);
+  T *Pz = split ? static_cast<T *>(p.part) + (int64_t)blockIdx.y * p.m * p.n : nullptr;
   const T ar = (T)p.alpha_re, ai = (T)p.alpha_im;
   const int fk = lane >> 4;
 #pragma unroll
@@ -264,6 +273,8 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
             if (!p.beta_zero) { vr += C[2 * off]; vi += C[2 * off + 1]; }
             C[2 * off] = vr;
             C[2 * off + 1] = vi;
+          } else if (split) {
+            Pz[row + col * p.m] = acc[0][i][j][r];
           } else {
             T v = ar * acc[0][i][j][r];
             if (!p.beta_zero) v += C[off];
@@ -621,8 +632,25 @@ template <typename K>
 int launch_grid(K kern, const GemmParams &p, hipStream_t s)
 {
   const int nb = p.tiles_m * p.tiles_n;
-  hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s, p);
+  const int nz = p.part ? (int)((p.k + p.kchunk - 1) / p.kchunk) : 1;
+  hipLaunchKernelGGL(kern, dim3(nb, nz), dim3(256), 0, s, p);
   return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
+}
+
+// C = alpha * sum_z part[z] (+ C) for split-K partials of a real GEMM, summed
+// in FP64; C and the partials are m x n column-major with leading dimension m.
+template <typename T>
+__global__ __launch_bounds__(256) void k_splitk_sum(const T *__restrict__ part, int nz, int64_t mn,
+                                                    T *__restrict__ c, double alpha, int beta_zero)
+{
+  const int64_t st = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < mn; i += st) {
+    double v = 0.0;
+    for (int z = 0; z < nz; ++z) v += (double)part[(int64_t)z * mn + i];
+    v *= alpha;
+    if (!beta_zero) v += (double)c[i];
+    c[i] = (T)v;
+  }
 }
 
 template <typename T, bool CPLX, int BM, int BN, int BK, int OCC>
@@ -699,9 +727,50 @@ int launch_bf16x3(const GemmArgs &g, void *ws, hipStream_t s)
 
 }  // namespace
 
+int gemm_default_tile(int kind, R4Mode r4mode, size_t m, size_t n, size_t k);
+
+int current_cus()
+{
+  static const int cus = []() {
+    int dev = 0, c = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+        prop.multiProcessorCount > 0)
+      c = prop.multiProcessorCount;
+    return c;
+  }();
+  return cus;
+}
+
+// Split-K of a real FP64/FP32 GEMM on 64x64 tiles: when the output has fewer
+// tiles than two resident workgroups per CU, K is cut into parts of >= 256
+// k-values (multiples of the 16-deep k-step) so that the grid fills the chip;
+// the partials are summed in FP64 by k_splitk_sum. Returns the number of
+// parts (1: no split) and the chunk length.
+int splitk_parts(const GemmArgs &g, int tile, int64_t *kchunk)
+{
+  *kchunk = (int64_t)g.k;
+  const bool real = g.kind == R8 || (g.kind == R4 && g.r4mode == R4Mode::kNativeF32);
+  if (!real || tile != 2 || g.k < 512) return 1;
+  const int64_t tiles = (int64_t)((g.m + 63) / 64) * (int64_t)((g.n + 63) / 64);
+  const int64_t want = 2 * (int64_t)current_cus();
+  if (tiles >= want) return 1;
+  int64_t nz = std::min<int64_t>({8, want / tiles, (int64_t)g.k / 256});
+  if (nz < 2) return 1;
+  const int64_t chunk = ((int64_t)g.k + nz - 1) / nz;
+  *kchunk = (chunk + 15) / 16 * 16;
+  return (int)(((int64_t)g.k + *kchunk - 1) / *kchunk);
+}
+
 size_t gemm_workspace_bytes(const GemmArgs &g)
 {
-  if (g.kind != R4 || g.r4mode != R4Mode::kBF16x3) return 0;
+  if (g.kind != R4 || g.r4mode != R4Mode::kBF16x3) {
+    if (g.kind == C8 || g.kind == C4) return 0;
+    const int tile = g.tile < 0 ? gemm_default_tile(g.kind, g.r4mode, g.m, g.n, g.k) : g.tile;
+    int64_t kc;
+    const int nz = splitk_parts(g, tile, &kc);
+    return nz > 1 ? (size_t)nz * g.m * g.n * (size_t)kind_size(g.kind) + 16 : 0;
+  }
   const B3Geom G = b3_geom(g);
   // split planes (bf16) + K-chunk partials (FP32, m x n each)
   return (size_t)(3 * (G.plane_a + G.plane_b)) * sizeof(unsigned short) +
@@ -715,14 +784,7 @@ int gemm_tile_choices(int kind, R4Mode r4mode);
 int gemm_default_tile(int kind, R4Mode r4mode, size_t m, size_t n, size_t k)
 {
   if (gemm_tile_choices(kind, r4mode) == 1) return 0;
-  static const int cus = []() {
-    int dev = 0, c = 256;
-    hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
-        prop.multiProcessorCount > 0)
-      c = prop.multiProcessorCount;
-    return c;
-  }();
+  const int cus = current_cus();
   // 128x128 tiles pay off only when there are enough of them to fill the
   // chip twice over and K is deep enough to amortise their prologue;
   // otherwise 64x64 tiles balance better (measured over random contractions)
@@ -767,6 +829,28 @@ int launch_gemm_ws(const GemmArgs &g, void *ws, hipStream_t s)
   }
   if ((p.m + 63) / 64 * ((p.n + 63) / 64) >= (1ll << 31)) return TALSH_LIMIT_EXCEEDED;
   const int tile = g.tile < 0 ? gemm_default_tile(g.kind, g.r4mode, g.m, g.n, g.k) : g.tile;
+  {
+    int64_t kc;
+    const int nz = ws ? splitk_parts(g, tile, &kc) : 1;
+    if (nz > 1) {
+      p.part = ws;
+      p.kchunk = kc;
+      const int64_t mn = (int64_t)g.m * (int64_t)g.n;
+      const unsigned rb = (unsigned)std::min<int64_t>((mn + 255) / 256, 4096);
+      int e;
+      if (g.kind == R8) {
+        e = launch_family<double, false, 64, 64, 16, 2>(p, g.a_kfast, g.b_kfast, s);
+        if (!e) hipLaunchKernelGGL(k_splitk_sum<double>, dim3(rb), dim3(256), 0, s, static_cast<const double *>(ws), nz,
+                                   mn, static_cast<double *>(g.c), g.alpha_re, p.beta_zero);
+      } else {
+        e = launch_family<float, false, 64, 64, 16, 2>(p, g.a_kfast, g.b_kfast, s);
+        if (!e) hipLaunchKernelGGL(k_splitk_sum<float>, dim3(rb), dim3(256), 0, s, static_cast<const float *>(ws), nz,
+                                   mn, static_cast<float *>(g.c), g.alpha_re, p.beta_zero);
+      }
+      if (e) return e;
+      return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
+    }
+  }
   switch (g.kind) {
   case R8:
     switch (tile) {

@@ -60,7 +60,8 @@ struct GemmArgs {
   int tile;  // tile-shape choice (-1: default)
 };
 int launch_gemm(const GemmArgs &g, hipStream_t s);
-// Bytes of scratch needed by launch_gemm (BF16x3 split planes); 0 otherwise.
+// Bytes of scratch needed by launch_gemm_ws: BF16x3 split planes and K-chunk
+// partials, or the partials of a split-K real product; 0 otherwise.
 size_t gemm_workspace_bytes(const GemmArgs &g);
 int launch_gemm_ws(const GemmArgs &g, void *ws, hipStream_t s);
 // Number of tile-shape choices of launch_gemm for a kind, and the output tile

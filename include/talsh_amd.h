@@ -46,8 +46,10 @@ int talshAmdPermutePlan(int data_kind, int rank, const int *dims, const int *o2n
  * C(m x n, column-major) = alpha * op(A) * op(B) (+ C unless beta_zero);
  * A is k x m k-fastest (a_kfast) or m x k m-fastest, B is k x n k-fastest
  * (b_kfast) or n x k n-fastest. fast_math selects the three-way BF16 split
- * for R4, which needs talshAmdGemmWorkspace() bytes of device workspace.
- * Synchronous; used by the autotuner and the tests. */
+ * for R4, which needs talshAmdGemmWorkspace() bytes of device workspace; real
+ * FP64/FP32 products with few output tiles split K when given that workspace
+ * (talshAmdGemmWorkspace covers every tile choice). Synchronous; used by the
+ * autotuner and the tests. */
 int talshAmdGemm(int data_kind, int fast_math, size_t m, size_t n, size_t k, const void *a, int a_kfast,
                  const void *b, int b_kfast, void *c, double alpha_re, double alpha_im, int beta_zero, int tile,
                  void *workspace, size_t workspace_bytes);

@@ -1,7 +1,9 @@
 """Matrix kernels on awkward shapes through talshAmdGemm: the three-way BF16
 split (tile-blocked operands, split K) against FP64, ragged m/n/k, every
 operand orientation, beta 0/1 and K chunk edges; the tile argument must not
-change the three-way split's result."""
+change the three-way split's result. Real FP64/FP32 products with few output
+tiles split K into FP64-summed partials; with and without that workspace they
+agree with FP64."""
 import numpy as np
 import pytest
 
@@ -11,7 +13,7 @@ from gpu_util import engine, gpu_tensor, rel_err
 pytestmark = pytest.mark.gpu
 
 
-def _gemm(m, n, k, ak, bk, beta_zero, tile, fast, kind=T.R4, seed=0):
+def _gemm(m, n, k, ak, bk, beta_zero, tile, fast, kind=T.R4, seed=0, workspace=True):
     eng = engine()
     L = eng.lib
     rng = np.random.default_rng(seed)
@@ -26,7 +28,7 @@ def _gemm(m, n, k, ak, bk, beta_zero, tile, fast, kind=T.R4, seed=0):
     if not beta_zero:
         want += c0.reshape(n, m).T.astype(np.float64)
     ta, tb, tc = gpu_tensor(a, [a.size]), gpu_tensor(b, [b.size]), gpu_tensor(c0, [c0.size])
-    wsb = L.talshAmdGemmWorkspace(kind, fast, m, n, k)
+    wsb = L.talshAmdGemmWorkspace(kind, fast, m, n, k) if workspace else 0
     tw = T.Tensor(L, T.R4, [max(1, (wsb + 3) // 4)], eng.gpu(0), 0.0) if wsb else None
     ptr = lambda t: t.body(eng.gpu(0))[1] if t is not None else None
     e = L.talshAmdGemm(kind, fast, m, n, k, ptr(ta), ak, ptr(tb), bk, ptr(tc), alpha, 0.0, 1 if beta_zero else 0,
@@ -61,3 +63,28 @@ def test_native_kernels_ragged(kind, tile):
     e, got, want = _gemm(333, 177, 1001, 1, 0, False, tile, fast=0, kind=kind, seed=3)
     assert e == 0
     assert rel_err(got, want) <= (1e-5 if kind == T.R4 else 1e-13)
+
+
+@pytest.mark.parametrize("kind", [T.R8, T.R4])
+@pytest.mark.parametrize("m,n,k,ak,bk,beta_zero,tile", [
+    (256, 192, 5000, 1, 1, True, 2),     # 12 tiles: K in 8 parts
+    (1000, 700, 2100, 0, 1, False, 2),   # 176 tiles: 2 parts, accumulate
+    (64, 64, 513, 1, 0, False, -1),      # one tile, default choice: 2 parts
+    (1024, 1024, 1024, 1, 1, True, -1),  # the extent-32 rank-4 contraction's product: 2 parts
+    (37, 5, 4099, 0, 0, True, 2),        # ragged everything
+])
+@pytest.mark.parametrize("workspace", [True, False])
+def test_real_split_k(kind, m, n, k, ak, bk, beta_zero, tile, workspace):
+    e, got, want = _gemm(m, n, k, ak, bk, beta_zero, tile, fast=0, kind=kind, seed=m + n, workspace=workspace)
+    assert e == 0
+    assert rel_err(got, want) <= (1e-5 if kind == T.R4 else 1e-13)
+
+
+def test_split_k_workspace_is_reported():
+    L = engine().lib
+    # few output tiles, deep K: partials for up to 8 parts
+    assert L.talshAmdGemmWorkspace(T.R8, 0, 256, 192, 5000) == 8 * 256 * 192 * 8 + 16
+    assert L.talshAmdGemmWorkspace(T.R4, 0, 256, 192, 5000) == 8 * 256 * 192 * 4 + 16
+    # many tiles or complex: none
+    assert L.talshAmdGemmWorkspace(T.R8, 0, 9216, 9216, 16384) == 0
+    assert L.talshAmdGemmWorkspace(T.C8, 0, 64, 64, 5000) == 0
