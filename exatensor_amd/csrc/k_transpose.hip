@@ -571,6 +571,9 @@ int plan(int es, int rank, const int *dims, const int *o2n, Planned &pl, int cus
   if (f.rank > kMaxGridDims) { pl.path = 2; return 0; }
   uint64_t vol = 1;
   for (int e : f.ext) vol *= (uint64_t)e;
+  // a small tensor gets tiles small enough for about four per CU (a tile
+  // takes one workgroup and the grid would otherwise leave CUs idle)
+  const long min_vol = std::min<long>(kTileMinVol, std::max<long>(1024, (long)(vol / (4 * (uint64_t)std::max(cus, 1)))));
   // two attempts: first with splits that keep runs whole cache lines, then
   // without, if the aligned tile would need more vectors per thread than the
   // kernel is instantiated for
@@ -590,7 +593,7 @@ int plan(int es, int rank, const int *dims, const int *o2n, Planned &pl, int cus
       target = std::max(1, target / 2);
     }
     TilePlan g = tp;
-    grow_tile(f, g, std::min(kTileMinVol, max_vol), max_vol);
+    grow_tile(f, g, std::min(min_vol, max_vol), max_vol);
     int gd = 0;
     for (int x : g.t) gd += x > 0;
     if (gd <= kMaxTileDims) tp = g;
