@@ -105,3 +105,19 @@ def test_configured_shapes_bit_exact(oracle, rank, ext, kind):
             rng.shuffle(perm)
         got = _gpu_copy(x, dims, perm)
         assert np.array_equal(got, _numpy_perm(x, dims, perm)), perm
+
+
+@pytest.mark.parametrize("kind", [T.R4, T.R8, T.C4, T.C8])
+@pytest.mark.parametrize("n", [1, 3, 64, 1000, 4099, 262147])
+def test_identity_copies_bit_exact(oracle, kind, n):
+    """Identity permutations take the flat copy (16-byte bodies: one-wave
+    workgroups, a partial last workgroup when the vector count is not a
+    multiple of 64) or the element copy (other sizes); conjugated complex
+    copies flip the imaginary signs."""
+    nrs = np.random.default_rng(n + kind)
+    x = nrs.uniform(-1, 1, n) + (1j * nrs.uniform(-1, 1, n) if kind in (T.C4, T.C8) else 0)
+    x = x.astype(T.NP_DTYPE[kind])
+    for conj in ((False, True) if kind in (T.C4, T.C8) else (False,)):
+        got = _gpu_copy(x, [n], [0], conj)
+        exp = oracle.permute([n], [1], x, 1 if conj else 0)
+        assert np.array_equal(got.view(np.uint8), exp.view(np.uint8)), (n, conj)
