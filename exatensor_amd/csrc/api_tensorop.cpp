@@ -1,10 +1,15 @@
 // talsh.h tensor-operation objects (TALSH/talshc.cpp:2598-3300).
 //
 // The object lifecycle, argument/specification setters, the flop/byte
-// accounting and the halving decomposition (used by talshTensorContractXL)
-// are provided. The staged execution API (activate / load input / execute /
-// store output / progress) reports TALSH_NOT_IMPLEMENTED: the out-of-memory
-// driver of this engine executes its pieces directly.
+// accounting, the halving decomposition (used by talshTensorContractXL) and
+// the staged execution of a contraction on argument slices: activate
+// (construct one tensor per argument slice on the execution GPU), load input
+// (slice L and R out of their tensors), execute (asynchronous contraction into
+// the D slice), test, store output (accumulative insert of the D slice into its
+// tensor), deactivate, and the progress state machine over those stages
+// (TALSH/talshc.cpp:2708-3000). As in the reference, only contractions
+// execute.
+#include <chrono>
 #include <vector>
 #include <cstring>
 
@@ -72,20 +77,183 @@ int talshTensorOpSetExecDevice(talsh_tens_op_t *op, int dev_id, int dev_kind)
   return 0;
 }
 
-int talshTensorOpActivate(talsh_tens_op_t *) { return TALSH_NOT_IMPLEMENTED; }
-int talshTensorOpLoadInput(talsh_tens_op_t *) { return TALSH_NOT_IMPLEMENTED; }
-int talshTensorOpExecute(talsh_tens_op_t *, int, int) { return TALSH_NOT_IMPLEMENTED; }
-int talshTensorOpTest(talsh_tens_op_t *, int *completed, int)
+}  // extern "C"
+
+namespace {
+
+double wall_seconds()
 {
-  if (completed) *completed = NOPE;
-  return TALSH_NOT_IMPLEMENTED;
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
-int talshTensorOpStoreOutput(talsh_tens_op_t *) { return TALSH_NOT_IMPLEMENTED; }
-int talshTensorOpDeactivate(talsh_tens_op_t *) { return TALSH_NOT_IMPLEMENTED; }
-int talshTensorOpProgress(talsh_tens_op_t *, int *done)
+
+// Flat id of the GPU that holds the argument slices: the preset execution
+// device when it is a GPU, else GPU 0.
+int slice_device(const talsh_tens_op_t *op)
 {
-  if (done) *done = NOPE;
-  return TALSH_NOT_IMPLEMENTED;
+  int kind = DEV_NULL;
+  if (op->exec_dev_id != DEV_NULL && talshKindDevId(op->exec_dev_id, &kind) >= 0 && kind != DEV_HOST)
+    return op->exec_dev_id;
+  return talshFlatDevId(DEV_AMD_GPU, 0);
+}
+
+// Integer base offsets of an argument slice.
+std::vector<int> slice_offsets(const talsh_tens_slice_t &sl)
+{
+  std::vector<int> offs(std::max(sl.shape.num_dim, 0));
+  for (size_t j = 0; j < offs.size(); ++j) offs[j] = sl.bases.offsets ? (int)sl.bases.offsets[j] : 0;
+  return offs;
+}
+
+}  // namespace
+
+extern "C" {
+
+int talshTensorOpActivate(talsh_tens_op_t *op)
+{
+  if (!op) return TALSH_INVALID_ARGS;
+  if (op->opkind == TALSH_TENSOR_NOOP || op->stage != TALSH_OP_DEFINED) return TALSH_NOT_ALLOWED;
+  op->time_started = wall_seconds();
+  const int dev = slice_device(op);
+  int e = 0;
+  unsigned i = 0;
+  for (; i < op->num_args && !e; ++i) {
+    const talsh_tens_slice_t &sl = op->tens_slice[i];
+    talsh_tens_t *t = &op->tens_arg[i];
+    e = talshTensorClean(t);
+    // slice bodies are created uninitialised: L and R are loaded, D is
+    // overwritten by the contraction
+    if (!e) e = talshTensorConstruct(t, op->data_kind, sl.shape.num_dim, sl.shape.dims, dev, nullptr, -1, nullptr,
+                                     0.0, 0.0);
+  }
+  if (e) {  // leave the operation as it was
+    for (unsigned j = 0; j < i; ++j)
+      if (talshTensorIsEmpty(&op->tens_arg[j]) != YEP) talshTensorDestruct(&op->tens_arg[j]);
+    return e;
+  }
+  op->stage = TALSH_OP_RESOURCED;
+  return 0;
+}
+
+int talshTensorOpLoadInput(talsh_tens_op_t *op)
+{
+  if (!op) return TALSH_INVALID_ARGS;
+  if (op->stage != TALSH_OP_RESOURCED) return TALSH_NOT_ALLOWED;
+  const int dev = slice_device(op);
+  for (unsigned i = 1; i < op->num_args; ++i) {  // input slices only
+    talsh_tens_slice_t &sl = op->tens_slice[i];
+    if (talshTensorRank(sl.tensor) != talshTensorRank(&op->tens_arg[i])) return TALSH_OBJECT_BROKEN;
+    const std::vector<int> offs = slice_offsets(sl);
+    const int e = talshTensorSlice(&op->tens_arg[i], sl.tensor, offs.data(), dev, DEV_DEFAULT, COPY_MT, NOPE, nullptr);
+    if (e) return e;
+  }
+  op->stage = TALSH_OP_LOADED;
+  return 0;
+}
+
+int talshTensorOpExecute(talsh_tens_op_t *op, int dev_id, int dev_kind)
+{
+  if (!op) return TALSH_INVALID_ARGS;
+  if (op->stage != TALSH_OP_LOADED) return TALSH_NOT_ALLOWED;
+  // the device: preset in the operation (then the call passes defaults), or
+  // given here (and recorded)
+  if (op->exec_dev_id != DEV_NULL) {
+    if (dev_id != DEV_DEFAULT || dev_kind != DEV_DEFAULT) return TALSH_INVALID_ARGS;
+    dev_id = talshKindDevId(op->exec_dev_id, &dev_kind);
+  } else {
+    if (dev_id == DEV_DEFAULT) return TALSH_INVALID_ARGS;
+    if (dev_kind == DEV_DEFAULT) {
+      op->exec_dev_id = dev_id;
+      dev_id = talshKindDevId(op->exec_dev_id, &dev_kind);
+    } else {
+      op->exec_dev_id = talshFlatDevId(dev_kind, dev_id);
+    }
+  }
+  if (op->opkind != TALSH_TENSOR_CONTRACT) return TALSH_NOT_IMPLEMENTED;
+  if (op->num_args != 3 || !op->symb_pattern) return TALSH_INVALID_ARGS;
+  op->time_scheduled = wall_seconds();
+  const int e = talshTensorContract(op->symb_pattern, &op->tens_arg[0], &op->tens_arg[1], &op->tens_arg[2],
+                                    op->alpha_real, op->alpha_imag, dev_id, dev_kind, COPY_TTT, NOPE,
+                                    &op->task_handle);
+  if (e) return e;
+  op->stage = TALSH_OP_SCHEDULED;
+  return 0;
+}
+
+int talshTensorOpTest(talsh_tens_op_t *op, int *completed, int wait)
+{
+  if (!op || !completed) return TALSH_INVALID_ARGS;
+  *completed = NOPE;
+  if (op->stage != TALSH_OP_SCHEDULED) return TALSH_NOT_ALLOWED;
+  int sts = TALSH_TASK_EMPTY, e = 0;
+  if (wait == YEP) {
+    e = talshTaskWait(&op->task_handle, &sts);
+    if (!e && sts == TALSH_TASK_COMPLETED) *completed = YEP;
+  } else {
+    const int ans = talshTaskComplete(&op->task_handle, &sts, &e);
+    if (!e && ans == YEP && sts == TALSH_TASK_COMPLETED) *completed = YEP;
+  }
+  if (!e && sts == TALSH_TASK_ERROR) e = TALSH_TASK_ERROR;
+  if (!e && *completed == YEP) {
+    op->time_completed = wall_seconds();
+    op->stage = TALSH_OP_COMPLETED;
+  }
+  return e;
+}
+
+int talshTensorOpStoreOutput(talsh_tens_op_t *op)
+{
+  if (!op) return TALSH_INVALID_ARGS;
+  if (op->stage != TALSH_OP_COMPLETED) return TALSH_NOT_ALLOWED;
+  if (op->num_args > 0) {
+    talsh_tens_slice_t &sl = op->tens_slice[0];
+    if (talshTensorRank(sl.tensor) != talshTensorRank(&op->tens_arg[0])) return TALSH_OBJECT_BROKEN;
+    const std::vector<int> offs = slice_offsets(sl);
+    const int e = talshTensorInsert(sl.tensor, &op->tens_arg[0], offs.data(), slice_device(op), DEV_DEFAULT, COPY_MT,
+                                    YEP, nullptr);  // accumulative insert
+    if (e) return e;
+  }
+  op->stage = TALSH_OP_STORED;
+  return 0;
+}
+
+int talshTensorOpDeactivate(talsh_tens_op_t *op)
+{
+  if (!op) return TALSH_INVALID_ARGS;
+  if (op->stage != TALSH_OP_RESOURCED && op->stage != TALSH_OP_STORED) return TALSH_NOT_ALLOWED;
+  int ier = 0, e = 0;
+  if (op->stage == TALSH_OP_STORED) ier = talshTaskDestruct(&op->task_handle);
+  for (int i = (int)op->num_args - 1; i >= 0 && !e; --i)
+    if (talshTensorIsEmpty(&op->tens_arg[i]) != YEP) e = talshTensorDestruct(&op->tens_arg[i]);
+  op->time_finished = wall_seconds();
+  if (e) return e;
+  op->stage = op->stage == TALSH_OP_STORED ? TALSH_OP_RETIRED : TALSH_OP_DEFINED;
+  return ier ? NOT_CLEAN : 0;
+}
+
+int talshTensorOpProgress(talsh_tens_op_t *op, int *done)
+{
+  if (!done) return TALSH_INVALID_ARGS;
+  *done = NOPE;
+  if (!op) return TALSH_INVALID_ARGS;
+  // synchronous stages run back to back; scheduling the contraction and a
+  // negative completion test yield to the caller
+  for (;;) {
+    int e = 0, completed = NOPE;
+    switch (op->stage) {
+    case TALSH_OP_DEFINED: e = talshTensorOpActivate(op); break;
+    case TALSH_OP_RESOURCED: e = talshTensorOpLoadInput(op); break;
+    case TALSH_OP_LOADED: return talshTensorOpExecute(op, DEV_DEFAULT, DEV_DEFAULT);
+    case TALSH_OP_SCHEDULED:
+      e = talshTensorOpTest(op, &completed, NOPE);
+      if (e || completed != YEP) return e;
+      break;
+    case TALSH_OP_COMPLETED: e = talshTensorOpStoreOutput(op); break;
+    case TALSH_OP_STORED: e = talshTensorOpDeactivate(op); break;
+    case TALSH_OP_RETIRED: *done = YEP; return 0;
+    default: return TALSH_NOT_ALLOWED;
+    }
+    if (e) return e;
+  }
 }
 // Splits a contraction in two along one dimension, halving it (the first half
 // gets the odd element): a batch index (in L, R and D) if any is larger than 1,
@@ -171,6 +339,7 @@ int talshTensorOpDestruct(talsh_tens_op_t *op)
 {
   if (!op) return TALSH_INVALID_ARGS;
   if (talshTaskIsEmpty(&op->task_handle) != YEP) {
+    // an operation whose contraction still runs cannot be destructed
     const int e = talshTaskDestruct(&op->task_handle);
     if (e) return e;
   }
