@@ -344,10 +344,12 @@ def run_selector(gb, n=300, seed=303):
     return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in rep.items()}
 
 
-def cpu_baseline(target_seconds=12.0):
+def cpu_baseline(target_seconds=12.0, contract_seconds=3.0):
     """The reference TAL-SH CPU path (oracle/_ref, built from the reference's
     own sources) timed on this host's cores for a bounded sample of the same
-    transposes; falls back to the oracle's C restatement if it is absent."""
+    transposes; falls back to the oracle's C restatement if it is absent.
+    With the reference build present, also times the extent-32 rank-4 R8
+    contraction on DEV_HOST (returned second, None otherwise)."""
     ncores = os.cpu_count() or 1
     os.environ.setdefault("OMP_NUM_THREADS", str(ncores))
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libtalsh_ref.so")
@@ -402,9 +404,46 @@ def cpu_baseline(target_seconds=12.0):
         total += 2 * vol * 8
         n += 1
     dt = time.perf_counter() - t0
-    return dict(value=round(total / 1e9 / dt, 3), unit="GB/s", cores=int(os.environ["OMP_NUM_THREADS"]),
+    base = dict(value=round(total / 1e9 / dt, 3), unit="GB/s", cores=int(os.environ["OMP_NUM_THREADS"]),
                 kind=kind, sample=f"{n} of the 64 rank-6 extent-24 R8 permutations via talshTensorCopy on DEV_HOST "
                                   f"({dt:.1f} s)")
+    c1 = None
+    if kind == "reference":
+        for t in (src, dst):
+            L.talshTensorDestruct(C.byref(t))
+        # D(a,b,c,d)+=L(d,i,b,j)*R(j,c,i,a), extents 32, on the reference's CPU path
+        L.talshTensorContract.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                          C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        d4 = (C.c_int * 4)(32, 32, 32, 32)
+        ct = [Tens() for _ in range(3)]
+        for t in ct:
+            L.talshTensorClean(C.byref(t))
+            assert L.talshTensorConstruct(C.byref(t), 8, 4, d4, 0, None, -1, None, 0.0, 0.0) == 0
+        rng = np.random.default_rng(34)
+        for t in ct[1:]:
+            v = rng.uniform(-1, 1, 32 ** 4)
+            L.talshTensorGetBodyAccess(C.byref(t), C.byref(p), 8, 0, 0)
+            C.memmove(p, v.ctypes.data, v.nbytes)
+        pat1 = b"D(a,b,c,d)+=L(d,i,b,j)*R(j,c,i,a)"
+
+        def contract():
+            e = L.talshTensorContract(pat1, C.byref(ct[0]), C.byref(ct[1]), C.byref(ct[2]), 1.0, 0.0, 0, 0, 26, 1,
+                                      None)
+            if e:
+                raise RuntimeError(f"CPU baseline contraction failed: {e}")
+        contract()
+        t0 = time.perf_counter()
+        reps = 0
+        while reps < 20 and (reps < 1 or time.perf_counter() - t0 < contract_seconds):
+            contract()
+            reps += 1
+        dt = time.perf_counter() - t0
+        c1 = dict(gflops=round(2.0 * 32 ** 6 * reps / dt / 1e9, 3), ms_per_op=round(dt / reps * 1e3, 3),
+                  cores=int(os.environ["OMP_NUM_THREADS"]), kind="reference",
+                  sample=f"{reps} blocking talshTensorContract calls on DEV_HOST ({dt:.1f} s)")
+        for t in ct:
+            L.talshTensorDestruct(C.byref(t))
+    return base, c1
 
 
 def main():
@@ -457,7 +496,9 @@ def main():
         "components": comps,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline()
+        line["cpu_baseline"], c1_cpu = cpu_baseline()
+        if c1_cpu and "c1_r8_ext32" in comps:
+            comps["c1_r8_ext32"]["cpu_reference"] = c1_cpu
     print(json.dumps(line), flush=True)
 
 
