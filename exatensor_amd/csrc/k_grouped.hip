@@ -225,20 +225,28 @@ __global__ __launch_bounds__(256, OCC) void k_grouped_gemm(const BlockOp *__rest
 // operands are three bf16 planes (plane stride rows * kp); each k-tile of 32
 // runs six 16x16x32 bf16 MFMA products per output block, and the FP32
 // accumulator is folded into an FP64 sum every 8 k-tiles (the chain's k-tiles
-// count together). One workgroup per CU (double-buffered planes fill the LDS).
-__global__ __launch_bounds__(256) void k_grouped_b3(const BlockOp *__restrict__ ops, const WorkItem *__restrict__ items,
-                                                    int nitems, int *__restrict__ counter, double alpha, int accumulate)
+// count together). One workgroup per CU (double-buffered planes fill the LDS)
+// of 8 waves, 2 x 4 over the tile (64 x 32 each), so that two waves share
+// each SIMD and one's MFMAs cover the other's LDS reads.
+constexpr int kB3GroupedThreads = 512;
+__global__ __launch_bounds__(kB3GroupedThreads) void k_grouped_b3(const BlockOp *__restrict__ ops,
+                                                                  const WorkItem *__restrict__ items, int nitems,
+                                                                  int *__restrict__ counter, double alpha,
+                                                                  int accumulate)
 {
+  constexpr int NT = kB3GroupedThreads;
   constexpr int BM = kGroupedTile, BN = kGroupedTile, BK = 32, KP = BK + 8;
-  constexpr int FM = BM / 32, FN = BN / 32;
-  constexpr int CA = 3 * BM * (BK / 8) / 256;  // 16-byte chunks of A (3 planes) per thread per k-tile
-  constexpr int CB = 3 * BN * (BK / 8) / 256;
+  constexpr int WM = BM / 2, WN = BN / 4;      // wave sub-tile
+  constexpr int FM = WM / 16, FN = WN / 16;    // 16x16 MFMA blocks per wave
+  constexpr int CA = 3 * BM * (BK / 8) / NT;   // 16-byte chunks of A (3 planes) per thread per k-tile
+  constexpr int CB = 3 * BN * (BK / 8) / NT;
+  static_assert(CA * NT == 3 * BM * (BK / 8) && CB * NT == 3 * BN * (BK / 8), "staging split");
   constexpr int kFold = 8;
   __shared__ __align__(16) unsigned short As[2][3][BM][KP];
   __shared__ __align__(16) unsigned short Bs[2][3][BN][KP];
   __shared__ int next_item;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * (BN / 2);
+  const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
   const int fr = lane & 15, fk = (lane >> 4) * 8;
   while (true) {
     if (tid == 0) next_item = atomicAdd(counter, 1);
@@ -269,13 +277,13 @@ __global__ __launch_bounds__(256) void k_grouped_b3(const BlockOp *__restrict__ 
       auto load = [&](int64_t k0) {
 #pragma unroll
         for (int i = 0; i < CA; ++i) {
-          const int c = tid + i * 256, pl = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
+          const int c = tid + i * NT, pl = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
           const int64_t row = m0 + r;
           ra[i] = row < op.m ? *reinterpret_cast<const us8 *>(A + pl * pa + row * kp + k0 + kc * 8) : us8{};
         }
 #pragma unroll
         for (int i = 0; i < CB; ++i) {
-          const int c = tid + i * 256, pl = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
+          const int c = tid + i * NT, pl = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
           const int64_t row = n0 + r;
           rb[i] = row < op.n ? *reinterpret_cast<const us8 *>(B + pl * pb + row * kp + k0 + kc * 8) : us8{};
         }
@@ -283,12 +291,12 @@ __global__ __launch_bounds__(256) void k_grouped_b3(const BlockOp *__restrict__ 
       auto store = [&](int buf) {
 #pragma unroll
         for (int i = 0; i < CA; ++i) {
-          const int c = tid + i * 256, pl = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
+          const int c = tid + i * NT, pl = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
           *reinterpret_cast<us8 *>(&As[buf][pl][r][kc * 8]) = ra[i];
         }
 #pragma unroll
         for (int i = 0; i < CB; ++i) {
-          const int c = tid + i * 256, pl = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
+          const int c = tid + i * NT, pl = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
           *reinterpret_cast<us8 *>(&Bs[buf][pl][r][kc * 8]) = rb[i];
         }
       };
@@ -381,7 +389,7 @@ int launch_grouped(int kind, bool split3, const BlockOp *ops, const WorkItem *it
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, counter);
   if (split3) {
     if (kind != R4) return TALSH_INVALID_ARGS;
-    hipLaunchKernelGGL(k_grouped_b3, dim3(grid), dim3(256), 0, s, ops, items, nitems, counter, alpha,
+    hipLaunchKernelGGL(k_grouped_b3, dim3(grid), dim3(kB3GroupedThreads), 0, s, ops, items, nitems, counter, alpha,
                        accumulate ? 1 : 0);
   } else if (kind == R8)
     hipLaunchKernelGGL((k_grouped_gemm<double, 2>), dim3(grid), dim3(256), 0, s, ops, items, nitems, counter, alpha,
