@@ -18,7 +18,7 @@ Follows ExaTENSOR's recursive tensors:
   * contraction_split: TensContractionSplitFunc
     (INTRAVIRT/tensor_recursive.F90:9202-9784) -- align the tree levels of
     dimensions that the pattern pairs (the finer side is promoted to its
-    ancestor at the coarser level, align_levels :9350-9440, judged on each
+    ancestor at the coarser level, align_levels :9350-9440, decided on each
     argument's first subtensor), describe every subtensor by its (promoted)
     subspace ids, join L and R on the contracted ids, form the Cartesian
     products, and keep those whose open ids name an existing D subtensor.
@@ -27,7 +27,7 @@ Follows ExaTENSOR's recursive tensors:
     contracted multi-index, then R subtensor, then L subtensor -- the order
     the reference's sorted merge produces.
 Every subtensor of an argument must sit at one tree level per dimension
-(the reference judges the alignment on the first subtensor and assumes
+(the reference decides the alignment on the first subtensor and assumes
 this; here a mix raises ValueError). Symmetric-dimension restrictions
 (get_dim_symmetry) are not supported: all dimensions are independent.
 
@@ -190,13 +190,13 @@ def contraction_split(pattern, dten, lten, rten):
             raise ValueError(f"tensor {t.name} has no subtensors (call split())")
     dr, lr, rr = dten.rank, lten.rank, rten.rank
     cp = _pattern_map(pattern, dr, lr, rr)
-    # the alignment is judged on the first subtensor, so every subtensor of
+    # the alignment is decided on the first subtensor, so every subtensor of
     # an argument must sit at one tree level per dimension
     for t in (dten, lten, rten):
         for i, sp in enumerate(t.spaces):
             if len({sp.level(s[i]) for s in t.subtensors}) > 1:
                 raise ValueError(f"subtensors of {t.name} mix tree levels in dimension {i}")
-    # level alignment, judged on the first subtensor of each argument
+    # level alignment, decided on the first subtensor of each argument
     adj = [[0] * dr, [0] * lr, [0] * rr]
     first = (dten.subtensors[0], lten.subtensors[0], rten.subtensors[0])
     tens = (dten, lten, rten)
