@@ -191,3 +191,29 @@ def test_device_id_codec(lib):
     assert lib.talshFlatDevId(T.DEV_AMD_GPU, 0) == 17
     k = C.c_int()
     assert lib.talshKindDevId(18, C.byref(k)) == 1 and k.value == T.DEV_AMD_GPU
+
+
+def test_split_k_workspace_rule(lib):
+    """Real FP64/FP32 products split K only when the 64x64 output tiles are
+    fewer than two per CU (256 CUs), into at most 8 parts of >= 256 k-values
+    (16-aligned); the workspace holds one m x n partial per part. BF16x3 and
+    complex workspaces follow their own rules. CPU only (the library's
+    device query falls back to 256 CUs without a GPU)."""
+    import random
+    rng = random.Random(4)
+    for _ in range(300):
+        kind = rng.choice([T.R8, T.R4])
+        es = 8 if kind == T.R8 else 4
+        m, n = rng.randint(1, 3000), rng.randint(1, 3000)
+        k = rng.randint(1, 40000)
+        ws = lib.talshAmdGemmWorkspace(kind, 0, m, n, k)
+        tiles = -(-m // 64) * -(-n // 64)
+        if ws == 0:
+            continue
+        nz, rem = divmod(ws - 16, m * n * es)
+        assert rem == 0 and 2 <= nz <= 8, (m, n, k, ws)
+        assert tiles < 512 and k >= 512, (m, n, k, nz)
+        chunk = -(-(-(-k // min(8, 512 // tiles, k // 256))) // 16) * 16
+        assert -(-k // chunk) == nz and chunk >= 256, (m, n, k, nz, chunk)
+    assert lib.talshAmdGemmWorkspace(T.R8, 0, 9216, 9216, 16384) == 0
+    assert lib.talshAmdGemmWorkspace(T.C8, 0, 64, 64, 5000) == 0
