@@ -123,7 +123,7 @@ def test_transpose_plans_fit_the_kernel(lib):
     with at most 8 vectors per thread, and the vector width divides 16 bytes.
     On the rank-6 extent-24 and rank-4 extent-128 workloads every plan is
     tiled, uses 16-byte vectors, and both contiguous runs are whole 128-byte
-    lines. Random shapes of ranks 2-8 and all four data kinds, CPU only."""
+    lines. Random shapes of ranks 2-40 and all four data kinds, CPU only."""
     import math
     import random
     es_of = {T.R4: 4, T.R8: 8, T.C4: 8, T.C8: 16}
@@ -149,6 +149,24 @@ def test_transpose_plans_fit_the_kernel(lib):
         assert tile * es <= 65536, (dims, perm, tile)
         assert -(-tile // (512 * vec)) <= 8, (dims, perm, tile, vec)
         assert vec * es <= 16 and grid >= 1
+    # ranks 8-40 of small extents: fused ranks above 16 take the per-element
+    # path; tiled plans keep the same limits
+    paths = set()
+    for _ in range(400):
+        kind = rng.choice(list(es_of))
+        es = es_of[kind]
+        rank = rng.randint(8, 40)
+        dims = [rng.choice((1, 2, 2, 3, 3, 4, 5)) for _ in range(rank)]
+        while math.prod(dims) > 1 << 22:
+            dims[dims.index(max(dims))] //= 2
+        perm = list(range(rank))
+        rng.shuffle(perm)
+        fused, path, tile, in_run, out_run, grid, vec = plan(kind, dims, perm)
+        paths.add(path)
+        assert path in (1, 2) and (fused <= 16 or path == 2), (dims, perm, fused, path)
+        if path == 1:
+            assert tile * es <= 65536 and -(-tile // (512 * vec)) <= 8 and vec * es <= 16, (dims, perm, tile)
+    assert paths == {1, 2}
     import bench
     for kind in (T.R4, T.R8):
         es = es_of[kind]
