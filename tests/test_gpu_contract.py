@@ -276,3 +276,37 @@ def test_arena_back_pressure():
     t = T.Tensor(L, T.R8, [int(cap // 8 // 4096) + 2, 4096], eng.gpu(0))
     assert t.err == T.DEVICE_UNABLE
     assert L.talshTensorIsEmpty(t.ptr) == T.YEP
+
+
+@pytest.mark.parametrize("kind", [T.R8, T.C8])
+def test_high_rank_contractions(oracle, kind):
+    """Random contractions of tensors of rank 4 to 14 with extents 1-4 (ranks
+    far above the configured ones: many unit extents, fusable and unfusable
+    index runs, free and contracted indices in random orders), against
+    long-double direct evaluation."""
+    rng = np.random.default_rng(140 + kind)
+    for case in range(12):
+        nl, nr, nc = (int(v) for v in rng.integers(2, 8, 3))
+        while nl + nc > 14 or nr + nc > 14:
+            nc -= 1
+        ext = {f"x{i}": int(rng.choice([1, 2, 2, 3, 3, 4])) for i in range(nl + nr + nc)}
+        names = list(ext)
+        fl, fr, cc = names[:nl], names[nl:nl + nr], names[nl + nr:]
+        for group in (fl, fr, cc):
+            while np.prod([ext[x] for x in group]) > 96:
+                ext[max(group, key=ext.get)] //= 2
+        dlab = list(rng.permutation(fl + fr))
+        llab = list(rng.permutation(fl + cc))
+        rlab = list(rng.permutation(cc + fr))
+        pat = f"D({','.join(dlab)})+=L({','.join(llab)})*R({','.join(rlab)})"
+        dd, ld, rd = ([ext[x] for x in lab] for lab in (dlab, llab, rlab))
+        vol = lambda dims: int(np.prod(dims))
+        def rand(n):
+            x = rng.uniform(-1, 1, n) + (1j * rng.uniform(-1, 1, n) if kind == T.C8 else 0)
+            return x.astype(T.NP_DTYPE[kind])
+        d, l, r = rand(vol(dd)), rand(vol(ld)), rand(vol(rd))
+        alpha = 0.75 - 0.5j if kind == T.C8 else -1.25
+        e, got = contract(pat, d, dd, l, ld, r, rd, alpha)
+        assert e == 0, (pat, e)
+        ex = oracle.contract_exact(pat, d, dd, l, ld, r, rd, alpha)
+        assert rel_err(got, ex) <= 1e-12, pat
