@@ -351,11 +351,12 @@ Fused fuse_dims(int rank, const int *dims, const int *o2n)
   std::sort(order.begin(), order.end(), [&](int a, int b) { return o2n[keep[a]] < o2n[keep[b]]; });
   std::vector<int> pos(keep.size());
   for (size_t q = 0; q < order.size(); ++q) pos[order[q]] = (int)q;
-  // fuse runs i, i+1 with pos[i+1] == pos[i] + 1
+  // fuse runs i, i+1 with pos[i+1] == pos[i] + 1 while the fused extent stays
+  // an int (a run left unfused is still a valid description of the copy)
   Fused f;
   std::vector<int> grp_pos;
   for (size_t i = 0; i < keep.size(); ++i) {
-    if (i > 0 && pos[i] == pos[i - 1] + 1) {
+    if (i > 0 && pos[i] == pos[i - 1] + 1 && (int64_t)f.ext.back() * dims[keep[i]] <= INT32_MAX) {
       f.ext.back() *= dims[keep[i]];
     } else {
       f.ext.push_back(dims[keep[i]]);

@@ -162,3 +162,32 @@ def test_high_rank_permutations_bit_exact(oracle, kind):
         exp = oracle.permute(dims, [p + 1 for p in perm], x, 1 if conj else 0)
         assert np.array_equal(got.view(np.uint8), exp.view(np.uint8)), (dims, perm, conj)
     assert {1, 2} <= paths, paths
+
+
+def test_wide_offsets_and_unfusable_runs():
+    """A transpose of 4.3 billion R4 elements (17 GB each way): element
+    offsets beyond 32 bits, and two leading extents whose fused extent would
+    leave the int range, so they stay separate dimensions. Every element
+    carries its input index (mod 2^32) as its bit pattern; 4M random output
+    positions and both ends of the output are checked against the
+    permutation's index formula."""
+    dims, perm = [65536, 32769, 2], [1, 2, 0]  # output dims (2, 65536, 32769)
+    n = 65536 * 32769 * 2
+    assert n >= 1 << 32
+    info = T.ints([0] * 7)
+    assert engine().lib.talshAmdPermutePlan(T.R4, 3, T.ints(dims), T.ints(perm), info) == 0
+    assert info[0] == 3 and info[1] == 1, list(info)  # unfused, tiled
+    x = np.empty(n, dtype=np.uint32)
+    step = 1 << 28
+    for s in range(0, n, step):
+        e = min(n, s + step)
+        x[s:e] = np.arange(s, e, dtype=np.uint64).astype(np.uint32)
+    got = _gpu_copy(x.view(np.float32), dims, perm).view(np.uint32)
+    del x
+    rng = np.random.default_rng(32)
+    o = np.concatenate([rng.integers(0, n, 1 << 22, dtype=np.uint64),
+                        np.arange(1 << 16, dtype=np.uint64), np.arange(n - (1 << 16), n, dtype=np.uint64)])
+    j2, t = o % 2, o // 2
+    j0, j1 = t % 65536, t // 65536
+    want = ((j0 + 65536 * (j1 + 32769 * j2)) & 0xFFFFFFFF).astype(np.uint32)
+    assert np.array_equal(got[o.astype(np.int64)], want)
