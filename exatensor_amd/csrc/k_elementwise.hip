@@ -207,13 +207,19 @@ double real_factor(double re, double im)
 // ---- box copy: tensor slice and insert ----------------------------------------
 // out[dbase + sum_i x_i ds_i] (+)= in[sbase + sum_i x_i ss_i] over the box
 // x < ext (dimension 0 fastest, its strides 1 on both sides after fusion).
+// One launch covers fewer than 2^31 elements (32-bit element decode); the host
+// cuts larger boxes into such pieces. Boxes of up to kBoxDims dimensions past
+// the first use the compact parameter block, others (up to the maximum rank)
+// the wide one.
 constexpr int kBoxDims = 8;
+constexpr int kBoxDimsWide = MAX_TENSOR_RANK;
 struct BoxDiv { uint32_t d, m, s; };
+template <int MAXD>
 struct BoxParams {
   int nd;                       // box dimensions after the first (row decode)
   BoxDiv e0;                    // extent of dimension 0
-  BoxDiv ext[kBoxDims];         // extents of dimensions 1..nd
-  int64_t ss[kBoxDims], ds[kBoxDims];
+  BoxDiv ext[MAXD];             // extents of dimensions 1..nd
+  int64_t ss[MAXD], ds[MAXD];
   int64_t sbase, dbase;
   uint64_t vol;
 };
@@ -223,12 +229,13 @@ __device__ __forceinline__ uint32_t bdiv(uint32_t n, const BoxDiv &f)
   return f.d == 1 ? n : ((__umulhi(n, f.m) + n) >> f.s);
 }
 
-template <typename R, int NC>
-__global__ __launch_bounds__(kNT) void k_box(const R *__restrict__ in, R *__restrict__ out, const BoxParams p, int acc)
+template <typename R, int NC, int MAXD>
+__global__ __launch_bounds__(kNT) void k_box(const R *__restrict__ in, R *__restrict__ out, const BoxParams<MAXD> p,
+                                             int acc)
 {
   const uint64_t st = (uint64_t)gridDim.x * kNT;
   for (uint64_t g = (uint64_t)blockIdx.x * kNT + threadIdx.x; g < p.vol; g += st) {
-    // g < 2^31 for 32-bit decode (checked by the launcher)
+    // g < 2^31 for 32-bit decode (the launcher cuts larger boxes)
     uint32_t r = bdiv((uint32_t)g, p.e0);
     const uint32_t x0 = (uint32_t)g - r * p.e0.d;
     int64_t so = p.sbase + x0, dof = p.dbase + x0;
@@ -247,35 +254,12 @@ __global__ __launch_bounds__(kNT) void k_box(const R *__restrict__ in, R *__rest
   }
 }
 
-}  // namespace
-
-int launch_box(const void *in, void *out, int kind, int rank, const int *box, const int64_t *sstr,
-               const int64_t *dstr, int64_t sbase, int64_t dbase, bool acc, hipStream_t s)
+template <int MAXD>
+int launch_box_piece(const void *in, void *out, int kind, const std::vector<int64_t> &e,
+                     const std::vector<int64_t> &a, const std::vector<int64_t> &b, int64_t sbase, int64_t dbase,
+                     uint64_t vol, bool acc, hipStream_t s)
 {
-  if (!kind_size(kind) || rank < 0) return TALSH_INVALID_ARGS;
-  // drop unit extents; fuse dimension i+1 into i when both sides are contiguous across them
-  std::vector<int64_t> e, a, b;
-  for (int i = 0; i < rank; ++i) {
-    if (box[i] <= 0) return TALSH_INVALID_ARGS;
-    if (box[i] == 1) continue;
-    if (!e.empty() && a.back() * e.back() == sstr[i] && b.back() * e.back() == dstr[i]) {
-      e.back() *= box[i];
-    } else {
-      e.push_back(box[i]);
-      a.push_back(sstr[i]);
-      b.push_back(dstr[i]);
-    }
-  }
-  uint64_t vol = 1;
-  for (int64_t x : e) vol *= (uint64_t)x;
-  if (e.empty()) { e.push_back(1); a.push_back(1); b.push_back(1); }
-  if (a[0] != 1 || b[0] != 1) {  // innermost box dimension not contiguous: keep it as a row dimension
-    e.insert(e.begin(), 1);
-    a.insert(a.begin(), 1);
-    b.insert(b.begin(), 1);
-  }
-  if ((int)e.size() - 1 > kBoxDims || vol >= (1ull << 31) || e[0] >= (1ll << 31)) return TALSH_LIMIT_EXCEEDED;
-  BoxParams p{};
+  BoxParams<MAXD> p{};
   auto mk = [](int64_t d) { FastDiv f((uint32_t)d); return BoxDiv{f.d, f.m, f.s}; };
   p.e0 = mk(e[0]);
   p.nd = (int)e.size() - 1;
@@ -287,17 +271,91 @@ int launch_box(const void *in, void *out, int kind, int rank, const int *box, co
   p.sbase = sbase;
   p.dbase = dbase;
   p.vol = vol;
-  if (vol == 0) return 0;
   const int blocks = (int)std::min<uint64_t>((vol + kNT - 1) / kNT, 65536);
   const int ac = acc ? 1 : 0;
   switch (kind) {
-  case R4: hipLaunchKernelGGL((k_box<float, 1>), dim3(blocks), dim3(kNT), 0, s, (const float *)in, (float *)out, p, ac); break;
-  case R8: hipLaunchKernelGGL((k_box<double, 1>), dim3(blocks), dim3(kNT), 0, s, (const double *)in, (double *)out, p, ac); break;
-  case C4: hipLaunchKernelGGL((k_box<float, 2>), dim3(blocks), dim3(kNT), 0, s, (const float *)in, (float *)out, p, ac); break;
-  case C8: hipLaunchKernelGGL((k_box<double, 2>), dim3(blocks), dim3(kNT), 0, s, (const double *)in, (double *)out, p, ac); break;
+  case R4: hipLaunchKernelGGL((k_box<float, 1, MAXD>), dim3(blocks), dim3(kNT), 0, s, (const float *)in, (float *)out, p, ac); break;
+  case R8: hipLaunchKernelGGL((k_box<double, 1, MAXD>), dim3(blocks), dim3(kNT), 0, s, (const double *)in, (double *)out, p, ac); break;
+  case C4: hipLaunchKernelGGL((k_box<float, 2, MAXD>), dim3(blocks), dim3(kNT), 0, s, (const float *)in, (float *)out, p, ac); break;
+  case C8: hipLaunchKernelGGL((k_box<double, 2, MAXD>), dim3(blocks), dim3(kNT), 0, s, (const double *)in, (double *)out, p, ac); break;
   default: return TALSH_INVALID_ARGS;
   }
   return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
+}
+
+// Launches the box (e, a, b) from the given bases, cut along its outermost
+// dimension (or along the row when the box is a single row) until every
+// piece has fewer than 2^31 elements.
+int launch_box_cut(const void *in, void *out, int kind, std::vector<int64_t> e, std::vector<int64_t> a,
+                   std::vector<int64_t> b, int64_t sbase, int64_t dbase, bool acc, hipStream_t s)
+{
+  constexpr uint64_t kMax = (1ull << 31) - 1;
+  uint64_t vol = 1;
+  for (int64_t x : e) vol *= (uint64_t)x;
+  if (vol <= kMax) {
+    if ((int)e.size() - 1 <= kBoxDims) return launch_box_piece<kBoxDims>(in, out, kind, e, a, b, sbase, dbase, vol, acc, s);
+    return launch_box_piece<kBoxDimsWide>(in, out, kind, e, a, b, sbase, dbase, vol, acc, s);
+  }
+  if (e.size() == 1) {  // one contiguous row: pieces of 2^30 elements
+    const int64_t step = 1ll << 30;
+    for (int64_t x = 0; x < e[0]; x += step) {
+      const int64_t n = std::min<int64_t>(step, e[0] - x);
+      const int err = launch_box_cut(in, out, kind, {n}, {1}, {1}, sbase + x, dbase + x, acc, s);
+      if (err) return err;
+    }
+    return 0;
+  }
+  const int64_t outer = e.back(), as = a.back(), bs = b.back();
+  const uint64_t inner = vol / (uint64_t)outer;
+  if (inner <= kMax) {  // several outer slabs per piece
+    const int64_t step = (int64_t)std::max<uint64_t>(1, kMax / inner);
+    for (int64_t x = 0; x < outer; x += step) {
+      std::vector<int64_t> ee = e;
+      ee.back() = std::min<int64_t>(step, outer - x);
+      const int err = launch_box_cut(in, out, kind, ee, a, b, sbase + x * as, dbase + x * bs, acc, s);
+      if (err) return err;
+    }
+    return 0;
+  }
+  e.pop_back(); a.pop_back(); b.pop_back();
+  for (int64_t x = 0; x < outer; ++x) {
+    const int err = launch_box_cut(in, out, kind, e, a, b, sbase + x * as, dbase + x * bs, acc, s);
+    if (err) return err;
+  }
+  return 0;
+}
+
+}  // namespace
+
+int launch_box(const void *in, void *out, int kind, int rank, const int *box, const int64_t *sstr,
+               const int64_t *dstr, int64_t sbase, int64_t dbase, bool acc, hipStream_t s)
+{
+  if (!kind_size(kind) || rank < 0 || rank > MAX_TENSOR_RANK) return TALSH_INVALID_ARGS;
+  // drop unit extents; fuse dimension i+1 into i when both sides are contiguous across them
+  // (a fused extent may pass 32 bits only in the contiguous row, which the
+  // launcher cuts; other extents are decoded in 32 bits)
+  std::vector<int64_t> e, a, b;
+  for (int i = 0; i < rank; ++i) {
+    if (box[i] <= 0) return TALSH_INVALID_ARGS;
+    if (box[i] == 1) continue;
+    const bool row = e.size() == 1 && a[0] == 1 && b[0] == 1;
+    if (!e.empty() && a.back() * e.back() == sstr[i] && b.back() * e.back() == dstr[i] &&
+        (row || e.back() * box[i] <= (int64_t)INT32_MAX)) {
+      e.back() *= box[i];
+    } else {
+      e.push_back(box[i]);
+      a.push_back(sstr[i]);
+      b.push_back(dstr[i]);
+    }
+  }
+  if (e.empty()) { e.push_back(1); a.push_back(1); b.push_back(1); }
+  if (a[0] != 1 || b[0] != 1) {  // innermost box dimension not contiguous: keep it as a row dimension
+    e.insert(e.begin(), 1);
+    a.insert(a.begin(), 1);
+    b.insert(b.begin(), 1);
+  }
+  if ((int)e.size() - 1 > kBoxDimsWide) return TALSH_LIMIT_EXCEEDED;
+  return launch_box_cut(in, out, kind, e, a, b, sbase, dbase, acc, s);
 }
 
 int launch_fill(void *d, int kind, size_t vol, double re, double im, hipStream_t s)
