@@ -955,6 +955,29 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
     double are, aim; int tile;
     void *lt = nullptr, *rt = nullptr, *dt = nullptr, *scr = nullptr, *tt = nullptr, *ws = nullptr;
     size_t wsb = 0;
+    // The matrix product as it is launched (operands exchanged when the plan
+    // swaps them). The workspace is sized from the same arguments: the
+    // three-way split pads rows of A and B to different multiples, so the
+    // two orientations need different amounts of scratch.
+    GemmArgs gemm_args(R4Mode r4mode) const
+    {
+      GemmArgs g{};
+      g.kind = D->kind;
+      g.r4mode = r4mode;
+      g.tile = tile;
+      g.beta_zero = !acc;
+      g.alpha_re = are;
+      g.alpha_im = kind_complex(D->kind) ? aim : 0.0;
+      if (!P.swap) {
+        g.m = P.m; g.n = P.n; g.conj_a = lconj; g.conj_b = rconj;
+      } else {
+        g.m = P.n; g.n = P.m; g.conj_a = rconj; g.conj_b = lconj;
+      }
+      g.k = P.k;
+      g.a_kfast = P.a_kfast;
+      g.b_kfast = P.b_kfast;
+      return g;
+    }
     int allocate(Temps &t)
     {
       const size_t es = kind_size(D->kind);
@@ -981,10 +1004,7 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
       if (!e && !P.r_o2n.empty()) e = t.alloc(tensor_volume(R->t) * es, &rt);
       if (!e && !P.d_in_o2n.empty()) e = t.alloc(tensor_volume(D->t) * es, &dt);
       if (!e) {
-        GemmArgs g{};
-        g.kind = D->kind; g.r4mode = t.d->r4mode; g.m = P.m; g.n = P.n; g.k = P.k;
-        g.tile = tile;  // the split-K workspace depends on the tile shape
-        wsb = gemm_workspace_bytes(g);
+        wsb = gemm_workspace_bytes(gemm_args(t.d->r4mode));
         if (wsb) e = t.alloc(wsb, &ws);
       }
       return e;
@@ -1044,24 +1064,13 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
         }
         dc = dt;
       }
-      GemmArgs g{};
-      g.kind = kind;
-      g.r4mode = dev->r4mode;
-      g.tile = tile;
-      g.beta_zero = !acc;
-      g.alpha_re = are;
-      g.alpha_im = kind_complex(kind) ? aim : 0.0;
-      if (!P.swap) {
-        g.m = P.m; g.n = P.n; g.k = P.k; g.a = la; g.b = rb; g.conj_a = lconj; g.conj_b = rconj;
-      } else {
-        g.m = P.n; g.n = P.m; g.k = P.k; g.a = rb; g.b = la; g.conj_a = rconj; g.conj_b = lconj;
-      }
-      g.a_kfast = P.a_kfast;
-      g.b_kfast = P.b_kfast;
+      GemmArgs g = gemm_args(dev->r4mode);
+      g.a = P.swap ? rb : la;
+      g.b = P.swap ? la : rb;
       g.c = dc;
       if (k->timed) (void)hipEventRecord(k->ev_mm0, s);
       prof_begin(s, kKGemm, pr);
-      int e = launch_gemm_ws(g, ws, s);
+      int e = launch_gemm_ws(g, ws, wsb, s);
       prof_end(s, kKGemm, pr);
       if (k->timed) (void)hipEventRecord(k->ev_mm1, s);
       k->has_mm = k->timed;
@@ -1561,12 +1570,13 @@ int talshAmdGemm(int data_kind, int fast_math, size_t m, size_t n, size_t k, con
   const size_t need = gemm_workspace_bytes(g);
   // the three-way split cannot run without its workspace; a real product
   // without room for split-K partials runs unsplit
-  if (g.r4mode == R4Mode::kBF16x3 && data_kind == R4 && need > workspace_bytes) return TALSH_INVALID_ARGS;
+  if (gemm_uses_split3(g) && need > workspace_bytes) return TALSH_INVALID_ARGS;
   (void)hipSetDevice(d->ordinal);
   hipStream_t s = d->streams[0];
   hipEvent_t pr[2];
   prof_begin(s, kKGemm, pr);
-  const int e = launch_gemm_ws(g, need <= workspace_bytes ? workspace : nullptr, s);
+  const int e = need <= workspace_bytes ? launch_gemm_ws(g, workspace, workspace_bytes, s)
+                                        : launch_gemm_ws(g, nullptr, 0, s);
   prof_end(s, kKGemm, pr);
   if (e) return e;
   return XT_HIP_OK(hipStreamSynchronize(s)) ? 0 : TALSH_FAILURE;

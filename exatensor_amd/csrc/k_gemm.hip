@@ -411,6 +411,60 @@ __global__ __launch_bounds__(256) void k_split3_rows(const float *__restrict__ x
   }
 }
 
+// Complex operands (C4) under the three-way split run as one real product of
+// the realified operands, with the contracted extent doubled (k' = 2k):
+//   A'' (2m x k'):  row 2r   = [ Re a(r,:), -Im a(r,:) ]
+//                   row 2r+1 = [ Im a(r,:),  Re a(r,:) ]
+//   B'' (n x k'):   row j    = [ Re b(j,:),  Im b(j,:) ]
+// so that (A'' B''^T)(2r, j) = Re c(r,j) and (2r+1, j) = Im c(r,j): the real
+// product's column-major 2m x n output is the complex m x n output,
+// interleaved. Conjugation negates the imaginary parts on the way in. The
+// pass writes the same tile-blocked planes as k_split3, one wave per 1 KB
+// block, reading the complex source element by element (k-fast or row-fast).
+template <int RB, int KB, bool IS_A>
+__global__ __launch_bounds__(256) void k_split3_cx(const float2 *__restrict__ x, int64_t rows, int64_t k, int64_t ld,
+                                                    int kfast, int conj, unsigned short *__restrict__ planes,
+                                                    int64_t rows_pad, int64_t k_pad, int64_t blk, int64_t plane)
+{
+  typedef B3Layout<RB, KB> Lay;
+  static_assert(RB * Lay::kChunks == 64, "one chunk per lane");
+  const int lane = threadIdx.x & 63;
+  const int64_t nkb = k_pad / KB;
+  const int64_t nblocks = rows_pad / RB * nkb;
+  const int rr = lane / Lay::kChunks;
+  const int c = Lay::pos(rr, lane % Lay::kChunks);
+  const int64_t wst = (int64_t)gridDim.x * 4;
+  const int64_t rrows = IS_A ? 2 * rows : rows;          // realified rows
+  const float isign = conj ? -1.f : 1.f;
+  for (int64_t bi = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); bi < nblocks; bi += wst) {
+    const int64_t band = bi / nkb, kb = bi % nkb;
+    const int64_t rp = band * RB + rr, k0 = kb * KB + c * 8;
+    const int64_t r = IS_A ? rp >> 1 : rp;
+    const int comp = IS_A ? (int)(rp & 1) : 0;
+    us8 h0, h1, h2;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int64_t kk2 = k0 + i;
+      float v = 0.f;
+      if (rp < rrows && kk2 < 2 * k) {
+        const int half = kk2 >= k ? 1 : 0;
+        const int64_t kk = kk2 - half * k;
+        const float2 z = x[kfast ? r * ld + kk : kk * ld + r];
+        const float zi = isign * z.y;
+        // A: (comp, half) = (0,0) re, (0,1) -im, (1,0) im, (1,1) re; B: (0,0) re, (0,1) im
+        v = (comp == half) ? z.x : (IS_A && half) ? -zi : zi;
+      }
+      unsigned short e0, e1, e2;
+      split3(v, e0, e1, e2);
+      h0[i] = e0; h1[i] = e1; h2[i] = e2;
+    }
+    const int64_t off = band * blk + kb * (RB * KB) + lane * 8;
+    *reinterpret_cast<us8 *>(planes + off) = h0;
+    *reinterpret_cast<us8 *>(planes + plane + off) = h1;
+    *reinterpret_cast<us8 *>(planes + 2 * plane + off) = h2;
+  }
+}
+
 struct B3Params {
   const unsigned short *a3;  // 3 tile-blocked planes of A, plane_a elements apart
   const unsigned short *b3;
@@ -628,6 +682,31 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(const float *__restrict__
   }
 }
 
+// Complex form (C4 under the three-way split): the partials hold the m x n
+// complex result interleaved; C = alpha * sum_z part[z] (+ C) with complex
+// alpha, summed in FP64.
+__global__ __launch_bounds__(256) void k_splitk_reduce_cx(const float2 *__restrict__ part, int nz, int64_t mn,
+                                                          float2 *__restrict__ c, double are, double aim,
+                                                          int beta_zero)
+{
+  const int64_t st = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < mn; i += st) {
+    double sr = 0.0, si = 0.0;
+    for (int z = 0; z < nz; ++z) {
+      const float2 v = part[(int64_t)z * mn + i];
+      sr += (double)v.x;
+      si += (double)v.y;
+    }
+    double orr = are * sr - aim * si, oi = are * si + aim * sr;
+    if (!beta_zero) {
+      const float2 cv = c[i];
+      orr += (double)cv.x;
+      oi += (double)cv.y;
+    }
+    c[i] = make_float2((float)orr, (float)oi);
+  }
+}
+
 template <typename K>
 int launch_grid(K kern, const GemmParams &p, hipStream_t s)
 {
@@ -672,11 +751,14 @@ struct B3Geom {
   int64_t kp, mp, np, blk, plane_a, plane_b, nz;
 };
 
+// Geometry of the split planes; a complex product runs realified (2m rows of
+// A, contracted extent 2k).
 B3Geom b3_geom(const GemmArgs &g)
 {
   B3Geom G;
-  G.kp = round_up(std::max<int64_t>((int64_t)g.k, 1), 32);
-  G.mp = round_up((int64_t)g.m, kB3M);
+  const int64_t cx = g.kind == C4 ? 2 : 1;
+  G.kp = round_up(std::max<int64_t>(cx * (int64_t)g.k, 1), 32);
+  G.mp = round_up(cx * (int64_t)g.m, kB3M);
   G.np = round_up((int64_t)g.n, kB3N);
   const int rb = B3Form<kB3Form>::RB;
   G.blk = rb * G.kp;
@@ -707,21 +789,42 @@ int launch_bf16x3(const GemmArgs &g, void *ws, hipStream_t s)
                          out, rows_pad, G.kp, G.blk, plane);
     }
   };
-  split(g.a, (int64_t)g.m, g.a_kfast, a3, G.mp, G.plane_a);
-  split(g.b, (int64_t)g.n, g.b_kfast, b3, G.np, G.plane_b);
+  if (g.kind == R4) {
+    split(g.a, (int64_t)g.m, g.a_kfast, a3, G.mp, G.plane_a);
+    split(g.b, (int64_t)g.n, g.b_kfast, b3, G.np, G.plane_b);
+  }
+  const bool cx = g.kind == C4;
+  if (cx) {
+    const int64_t nblocks_a = G.mp / F::RB * (G.kp / F::KB), nblocks_b = G.np / F::RB * (G.kp / F::KB);
+    const int ba = (int)std::min<int64_t>((nblocks_a + 3) / 4, 65536);
+    const int bb = (int)std::min<int64_t>((nblocks_b + 3) / 4, 65536);
+    const int64_t lda = g.a_kfast ? (int64_t)g.k : (int64_t)g.m, ldb = g.b_kfast ? (int64_t)g.k : (int64_t)g.n;
+    hipLaunchKernelGGL((k_split3_cx<F::RB, F::KB, true>), dim3(ba), dim3(256), 0, s,
+                       static_cast<const float2 *>(g.a), (int64_t)g.m, (int64_t)g.k, lda, g.a_kfast ? 1 : 0,
+                       g.conj_a ? 1 : 0, a3, G.mp, G.kp, G.blk, G.plane_a);
+    hipLaunchKernelGGL((k_split3_cx<F::RB, F::KB, false>), dim3(bb), dim3(256), 0, s,
+                       static_cast<const float2 *>(g.b), (int64_t)g.n, (int64_t)g.k, ldb, g.b_kfast ? 1 : 0,
+                       g.conj_b ? 1 : 0, b3, G.np, G.kp, G.blk, G.plane_b);
+  }
   B3Params p{};
   p.a3 = a3; p.b3 = b3;
   p.part = reinterpret_cast<float *>(b3 + 3 * G.plane_b);
-  p.m = (int64_t)g.m; p.n = (int64_t)g.n; p.k_pad = G.kp;
+  p.m = (cx ? 2 : 1) * (int64_t)g.m; p.n = (int64_t)g.n; p.k_pad = G.kp;
   p.kchunk = std::min<int64_t>(G.kp, kB3Chunk);
   p.blk = G.blk; p.plane_a = G.plane_a; p.plane_b = G.plane_b;
   p.tiles_m = (int)(G.mp / kB3M);
   p.tiles_n = (int)(G.np / kB3N);
   hipLaunchKernelGGL(k_gemm_bf16x3<kB3Form>, dim3(p.tiles_m * p.tiles_n, (int)G.nz), dim3(512), 0, s, p);
   const int64_t mn = (int64_t)g.m * (int64_t)g.n;
-  const int rb = (int)std::min<int64_t>((mn / 4 + 255) / 256 + 1, 8192);
-  hipLaunchKernelGGL(k_splitk_reduce, dim3(rb), dim3(256), 0, s, p.part, (int)G.nz, mn, static_cast<float *>(g.c),
-                     g.alpha_re, g.beta_zero ? 1 : 0);
+  if (cx) {
+    const int rb = (int)std::min<int64_t>((mn + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_splitk_reduce_cx, dim3(rb), dim3(256), 0, s, reinterpret_cast<const float2 *>(p.part),
+                       (int)G.nz, mn, static_cast<float2 *>(g.c), g.alpha_re, g.alpha_im, g.beta_zero ? 1 : 0);
+  } else {
+    const int rb = (int)std::min<int64_t>((mn / 4 + 255) / 256 + 1, 8192);
+    hipLaunchKernelGGL(k_splitk_reduce, dim3(rb), dim3(256), 0, s, p.part, (int)G.nz, mn, static_cast<float *>(g.c),
+                       g.alpha_re, g.beta_zero ? 1 : 0);
+  }
   return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
 }
 
@@ -764,7 +867,7 @@ int splitk_parts(const GemmArgs &g, int tile, int64_t *kchunk)
 
 size_t gemm_workspace_bytes(const GemmArgs &g)
 {
-  if (g.kind != R4 || g.r4mode != R4Mode::kBF16x3) {
+  if (!gemm_uses_split3(g)) {
     if (g.kind == C8 || g.kind == C4) return 0;
     const int tile = g.tile < 0 ? gemm_default_tile(g.kind, g.r4mode, g.m, g.n, g.k) : g.tile;
     int64_t kc;
@@ -772,12 +875,15 @@ size_t gemm_workspace_bytes(const GemmArgs &g)
     return nz > 1 ? (size_t)nz * g.m * g.n * (size_t)kind_size(g.kind) + 16 : 0;
   }
   const B3Geom G = b3_geom(g);
-  // split planes (bf16) + K-chunk partials (FP32, m x n each)
+  // split planes (bf16) + K-chunk partials (FP32, m x n each; complex: 2m x n)
+  const int64_t cx = g.kind == C4 ? 2 : 1;
   return (size_t)(3 * (G.plane_a + G.plane_b)) * sizeof(unsigned short) +
-         (size_t)(G.nz * (int64_t)g.m * (int64_t)g.n) * 4 + 16;
+         (size_t)(G.nz * cx * (int64_t)g.m * (int64_t)g.n) * 4 + 16;
 }
 
-int launch_gemm(const GemmArgs &g, hipStream_t s) { return launch_gemm_ws(g, nullptr, s); }
+int launch_gemm(const GemmArgs &g, hipStream_t s) { return launch_gemm_ws(g, nullptr, 0, s); }
+
+bool gemm_uses_split3(const GemmArgs &g) { return (g.kind == R4 || g.kind == C4) && g.r4mode == R4Mode::kBF16x3; }
 
 int gemm_tile_choices(int kind, R4Mode r4mode);
 
@@ -808,10 +914,14 @@ void gemm_tile_shape(int kind, R4Mode r4mode, int tile, int *bm, int *bn, int *o
   if (tile == 2) *bm = *bn = 64;
 }
 
-int launch_gemm_ws(const GemmArgs &g, void *ws, hipStream_t s)
+int launch_gemm_ws(const GemmArgs &g, void *ws, size_t ws_bytes, hipStream_t s)
 {
   if (g.m == 0 || g.n == 0) return 0;
   if (g.k == 0 && !g.beta_zero) return 0;  // empty contraction leaves C unchanged
+  if (ws && ws_bytes < gemm_workspace_bytes(g)) {
+    if (gemm_uses_split3(g)) return TALSH_INVALID_ARGS;
+    ws = nullptr;  // too small for split-K partials: unsplit
+  }
   GemmParams p{};
   p.a = g.a; p.b = g.b; p.c = g.c;
   p.m = (int64_t)g.m; p.n = (int64_t)g.n; p.k = (int64_t)g.k;
@@ -866,7 +976,9 @@ int launch_gemm_ws(const GemmArgs &g, void *ws, hipStream_t s)
     case 2: return launch_family<float, false, 64, 64, 16, 2>(p, g.a_kfast, g.b_kfast, s);
     default: return launch_family<float, false, 128, 128, 16, 2>(p, g.a_kfast, g.b_kfast, s);
     }
-  case C4: return launch_family<float, true, 64, 64, 16, 2>(p, g.a_kfast, g.b_kfast, s);
+  case C4:
+    if (g.r4mode == R4Mode::kBF16x3) return launch_bf16x3(g, ws, s);
+    return launch_family<float, true, 64, 64, 16, 2>(p, g.a_kfast, g.b_kfast, s);
   default: return TALSH_INVALID_ARGS;
   }
 }

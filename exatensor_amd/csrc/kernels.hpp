@@ -60,10 +60,15 @@ struct GemmArgs {
   int tile;  // tile-shape choice (-1: default)
 };
 int launch_gemm(const GemmArgs &g, hipStream_t s);
+// R4 and C4 products under fast math run as the three-way BF16 split (C4
+// realified: one real product over 2m x 2k and n x 2k operands).
+bool gemm_uses_split3(const GemmArgs &g);
 // Bytes of scratch needed by launch_gemm_ws: BF16x3 split planes and K-chunk
 // partials, or the partials of a split-K real product; 0 otherwise.
 size_t gemm_workspace_bytes(const GemmArgs &g);
-int launch_gemm_ws(const GemmArgs &g, void *ws, hipStream_t s);
+// ws holds ws_bytes; a workspace smaller than gemm_workspace_bytes(g) is an
+// error for the three-way split and runs a real product unsplit.
+int launch_gemm_ws(const GemmArgs &g, void *ws, size_t ws_bytes, hipStream_t s);
 // Number of tile-shape choices of launch_gemm for a kind, and the output tile
 // (bm x bn) and occupancy target of a choice.
 int gemm_tile_choices(int kind, R4Mode r4mode);

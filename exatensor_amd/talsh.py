@@ -88,6 +88,7 @@ def load_library(path=LIB_PATH):
         "talshQueryFastMath": (C.c_int, [C.c_int, C.c_int]),
         "talshDeviceBufferSize": (C.c_size_t, [C.c_int, C.c_int]),
         "talshDeviceBufferFreeSize": (C.c_size_t, [C.c_int, C.c_int]),
+        "talshDeviceTensorSize": (C.c_size_t, [C.c_int, C.c_int]),
         "talshTensorClean": (C.c_int, [_TP]),
         "talshTensorIsEmpty": (C.c_int, [_TP]),
         "talshTensorConstruct": (C.c_int, [_TP, C.c_int, C.c_int, _IP, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
@@ -109,6 +110,7 @@ def load_library(path=LIB_PATH):
                                           C.c_int, C.c_int, _KP]),
         "talshTaskClean": (C.c_int, [_KP]),
         "talshTaskStatus": (C.c_int, [_KP]),
+        "talshTaskIsEmpty": (C.c_int, [_KP]),
         "talshTaskComplete": (C.c_int, [_KP, _IP, _IP]),
         "talshTaskWait": (C.c_int, [_KP, _IP]),
         "talshTaskDestruct": (C.c_int, [_KP]),
@@ -235,9 +237,14 @@ class Tensor:
         e = self.lib.talshTensorGetBodyAccess(self.ptr, C.byref(p), self.kind, dev_id, DEV_NULL)
         return e, p.value
 
-    def download(self):
-        """Copies the body (from whichever image exists) into a numpy array."""
-        out = np.empty(self.volume, dtype=NP_DTYPE[self.kind])
+    def download(self, out=None):
+        """Copies the body (from whichever image exists) into a numpy array
+        (into `out` when given: a contiguous array of the body's size and
+        dtype, e.g. a view of pinned host memory)."""
+        if out is None:
+            out = np.empty(self.volume, dtype=NP_DTYPE[self.kind])
+        elif out.size != self.volume or out.dtype != NP_DTYPE[self.kind] or not out.flags.c_contiguous:
+            raise ValueError("download: output array does not match the body")
         n = self.t.ndev
         if n <= 0:
             raise RuntimeError("tensor has no body image")

@@ -7,6 +7,7 @@ import numpy as np
 from exatensor_amd import talsh as T
 
 _ENG = None
+_PINNED = {}
 
 
 def engine():
@@ -43,3 +44,17 @@ def contract(pattern, d, dd, l, ld, r, rd, alpha=1.0, acc=True, copy=T.COPY_MTT)
     for t in (td, tl, tr):
         t.destruct()
     return e, out
+
+
+def pinned(nbytes, dtype):
+    """A numpy array over pinned host memory (hipHostMalloc), reused per size
+    for the life of the process: device-to-host copies of multi-GB bodies run
+    at the link's rate instead of through pageable staging."""
+    if nbytes not in _PINNED:
+        hip = C.CDLL("libamdhip64.so")
+        hip.hipHostMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint]
+        hip.hipHostMalloc.restype = C.c_int
+        p = C.c_void_p()
+        assert hip.hipHostMalloc(C.byref(p), nbytes, 0) == 0, "hipHostMalloc failed"
+        _PINNED[nbytes] = (C.c_char * nbytes).from_address(p.value)
+    return np.frombuffer(_PINNED[nbytes], dtype=dtype)

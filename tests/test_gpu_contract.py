@@ -28,7 +28,7 @@ def golden():
 
 def test_golden_contractions(golden, oracle):
     meta, arrs = golden
-    n = 0
+    n = ndirect = 0
     for c in meta:
         if c["op"] != "contract" or c["ierr"] != 0:
             continue
@@ -42,7 +42,17 @@ def test_golden_contractions(golden, oracle):
                                       c["rdims"], alpha, c["accumulative"])
         assert rel_err(got, exact) <= _tol(c["kind"]), c
         n += 1
-    assert n > 150
+        # and directly against the output the reference's CPU path wrote,
+        # except where its -DNO_BLAS build mis-orders a conjugated R operand
+        e0, dig, dr, lr, rr, cj = oracle.str2dig(c["pattern"])
+        lconj, rconj = (cj >> 1) & 1, (cj >> 2) & 1
+        if cj & 1:
+            lconj, rconj = 1 - lconj, 1 - rconj
+        if c["kind"] in (T.C4, T.C8) and rconj and lr > 0 and rr > 0:
+            continue
+        assert rel_err(got, arrs[k + "_d"]) <= _tol(c["kind"]), c
+        ndirect += 1
+    assert n > 150 and ndirect > 100
 
 
 def test_golden_copies_adds_init_scale(golden):
@@ -310,3 +320,128 @@ def test_high_rank_contractions(oracle, kind):
         assert e == 0, (pat, e)
         ex = oracle.contract_exact(pat, d, dd, l, ld, r, rd, alpha)
         assert rel_err(got, ex) <= 1e-12, pat
+
+
+@pytest.mark.parametrize("pat,a,b,k", [
+    ("D(b,a)+=L(k,a)*R(k,b)", 256, 128, 200),   # planned with the operands exchanged
+    ("D(a,b)+=L(k,a)*R(k,b)", 256, 128, 200),
+    ("D(b,a)+=L(a,k)*R(b,k)", 130, 300, 96),
+    ("D(b,a)+=L(k,a)*R(k,b)", 520, 70, 4100),   # more than one K chunk candidate set, ragged rows
+])
+def test_fast_math_every_plan_variant_unequal_m_n(oracle, pat, a, b, k):
+    """R4 under the three-way BF16 split with m != n, through every
+    matricization the planner offers (some exchange the operands, so the
+    product runs as n x m): the split planes are padded differently along
+    rows of A and B, and the workspace must be sized for the orientation that
+    is launched (an undersized workspace is rejected by the launcher). Each
+    variant is held to 1e-6 of the FP64 evaluation."""
+    eng = engine()
+    L = eng.lib
+    import ctypes as C
+    lab = {"a": a, "b": b, "k": k}
+    sect = lambda s: s[s.index("(") + 1:s.index(")")].split(",")
+    dpart, rest = pat.split("+=")
+    lpart, rpart = rest.split("*")
+    dd, ld, rd = ([lab[x] for x in sect(p)] for p in (dpart, lpart, rpart))
+    l = oracle.uniform(T.R4, [int(np.prod(ld))], 21)
+    r = oracle.uniform(T.R4, [int(np.prod(rd))], 22)
+    d = oracle.uniform(T.R4, [int(np.prod(dd))], 23)
+    moved = (C.c_double * 64)()
+    nc = C.c_int()
+    assert L.talshAmdPlanCandidates(pat.encode(), 2, T.ints(dd), 2, T.ints(ld), 2, T.ints(rd), T.R4, 1, 64, moved,
+                                    C.byref(nc)) == 0
+    assert nc.value >= 2
+    for acc in (True, False):
+        ex = oracle.contract_exact(pat, d, dd, l, ld, r, rd, 0.5, acc)
+        assert L.talshEnableFastMath(T.DEV_AMD_GPU, 0) == 0
+        try:
+            for v in range(nc.value):
+                L.talshAmdSetPlanVariant(v, -1)
+                e, got = contract(pat, d, dd, l, ld, r, rd, 0.5, acc)
+                assert e == 0, (v, e)
+                assert rel_err(got, ex) <= 1e-6, (v, acc)
+        finally:
+            L.talshAmdSetPlanVariant(-1, -1)
+            L.talshDisableFastMath(T.DEV_AMD_GPU, 0)
+
+
+def test_config3_c8_full_size_sampled(oracle):
+    """The configured C8 contraction at full size, a=b=i=j=96, k=l=128, real
+    and imaginary parts N(0,1): 2048 sampled D elements against long-double
+    sums (each over 16384 complex products), <= 1e-12."""
+    pat = "D(a,b,i,j)+=L(a,k,i,l)*R(l,b,k,j)"
+    dd, ld, rd = [96] * 4, [96, 128, 96, 128], [128, 96, 128, 96]
+    l = oracle.normal(T.C8, int(np.prod(ld)), 31)
+    r = oracle.normal(T.C8, int(np.prod(rd)), 32)
+    d = np.zeros(96 ** 4, dtype=np.complex128)
+    e, got = contract(pat, d, dd, l, ld, r, rd, 1.0)
+    assert e == 0
+    pos = np.concatenate([np.random.default_rng(33).choice(d.size, 2048, replace=False),
+                          [0, d.size - 1]]).astype(np.uint64)
+    ex = oracle.contract_exact(pat, d, dd, l, ld, r, rd, 1.0, True, pos)
+    assert rel_err(got[pos.astype(np.int64)], ex) <= 1e-12
+
+
+@pytest.mark.parametrize("fast", [False, True])
+def test_config3_c4_modes_sampled(oracle, fast):
+    """The configured contraction in C4 at full size (a=b=i=j=96, k=l=128,
+    parts N(0,1)): native FP32 matrix cores (<= 1e-5) and the three-way BF16
+    split (<= 1e-6 against the long-double evaluation of the same complex64
+    inputs); 2048 sampled elements plus both ends."""
+    eng = engine()
+    L = eng.lib
+    pat = "D(a,b,i,j)+=L(a,k,i,l)*R(l,b,k,j)"
+    dd, ld, rd = [96] * 4, [96, 128, 96, 128], [128, 96, 128, 96]
+    l = oracle.normal(T.C4, int(np.prod(ld)), 35)
+    r = oracle.normal(T.C4, int(np.prod(rd)), 36)
+    d = np.zeros(96 ** 4, dtype=np.complex64)
+    if fast:
+        assert L.talshEnableFastMath(T.DEV_AMD_GPU, 0) == 0
+    try:
+        e, got = contract(pat, d, dd, l, ld, r, rd, 1.0)
+    finally:
+        L.talshDisableFastMath(T.DEV_AMD_GPU, 0)
+    assert e == 0
+    pos = np.concatenate([np.random.default_rng(37).choice(d.size, 2048, replace=False),
+                          [0, d.size - 1]]).astype(np.uint64)
+    ex = oracle.contract_exact(pat, d, dd, l, ld, r, rd, 1.0, True, pos)
+    err = rel_err(got[pos.astype(np.int64)], ex)
+    assert err <= (1e-6 if fast else 1e-5), err
+
+
+def test_c4_fast_math_conjugation_alpha_and_plans(oracle):
+    """C4 under fast math with conjugated operands (L+, R+, D+), complex
+    alpha, accumulating and not, through every plan variant (operands
+    exchanged in some): <= 1e-6 against long-double evaluation."""
+    import ctypes as C
+    eng = engine()
+    L = eng.lib
+    rng = np.random.default_rng(44)
+    for pat in ["D(a,b)+=L+(a,i)*R(i,b)", "D(a,b)+=L(a,i)*R+(i,b)", "D+(a,b)+=L(i,a)*R(b,i)",
+                "D(a,b,c)+=L+(c,i,a)*R+(b,i)", "D(b,a)+=L(k,a)*R(k,b)"]:
+        sect = lambda s: s[s.index("(") + 1:s.index(")")].split(",")
+        dpart, rest = pat.split("+=")
+        lpart, rpart = rest.split("*")
+        labs = [sect(x) for x in (dpart, lpart, rpart)]
+        hi = 300 if len(labs[0]) <= 2 else 40
+        ext = {x: int(rng.integers(20, hi)) for x in sorted(set(sum(labs, [])))}
+        dd, ld, rd = ([ext[x] for x in lab] for lab in labs)
+        mk = lambda n: (rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)).astype(np.complex64)
+        l, r, d = mk(int(np.prod(ld))), mk(int(np.prod(rd))), mk(int(np.prod(dd)))
+        moved = (C.c_double * 64)()
+        nc = C.c_int()
+        assert L.talshAmdPlanCandidates(pat.encode(), len(dd), T.ints(dd), len(ld), T.ints(ld), len(rd), T.ints(rd),
+                                        T.C4, 1, 64, moved, C.byref(nc)) == 0
+        alpha = complex(0.7, -0.4)
+        for acc in (True, False):
+            ex = oracle.contract_exact(pat, d, dd, l, ld, r, rd, alpha, acc)
+            assert L.talshEnableFastMath(T.DEV_AMD_GPU, 0) == 0
+            try:
+                for v in range(nc.value):
+                    L.talshAmdSetPlanVariant(v, -1)
+                    e, got = contract(pat, d, dd, l, ld, r, rd, alpha, acc)
+                    assert e == 0, (pat, v, e)
+                    assert rel_err(got, ex) <= 1e-6, (pat, v, acc)
+            finally:
+                L.talshAmdSetPlanVariant(-1, -1)
+                L.talshDisableFastMath(T.DEV_AMD_GPU, 0)

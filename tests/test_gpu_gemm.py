@@ -88,3 +88,60 @@ def test_split_k_workspace_is_reported():
     # many tiles or complex: none
     assert L.talshAmdGemmWorkspace(T.R8, 0, 9216, 9216, 16384) == 0
     assert L.talshAmdGemmWorkspace(T.C8, 0, 64, 64, 5000) == 0
+
+
+def _gemm_c4(m, n, k, ak, bk, beta_zero, fast, seed=0, alpha=0.75 - 0.4j):
+    eng = engine()
+    L = eng.lib
+    rng = np.random.default_rng(seed)
+    cx = lambda size: (rng.uniform(-1, 1, size) + 1j * rng.uniform(-1, 1, size)).astype(np.complex64)
+    a, b, c0 = cx(m * k), cx(n * k), cx(m * n)
+    A = a.reshape(m, k) if ak else a.reshape(k, m).T
+    B = b.reshape(n, k) if bk else b.reshape(k, n).T
+    want = alpha * (A.astype(np.complex128) @ B.astype(np.complex128).T)
+    if not beta_zero:
+        want += c0.reshape(n, m).T.astype(np.complex128)
+    ta, tb, tc = gpu_tensor(a, [a.size]), gpu_tensor(b, [b.size]), gpu_tensor(c0, [c0.size])
+    wsb = L.talshAmdGemmWorkspace(T.C4, fast, m, n, k)
+    tw = T.Tensor(L, T.R4, [max(1, (wsb + 3) // 4)], eng.gpu(0), 0.0) if wsb else None
+    ptr = lambda t: t.body(eng.gpu(0))[1] if t is not None else None
+    e = L.talshAmdGemm(T.C4, fast, m, n, k, ptr(ta), ak, ptr(tb), bk, ptr(tc), alpha.real, alpha.imag,
+                       1 if beta_zero else 0, -1, ptr(tw), wsb)
+    got = tc.download().reshape(n, m).T if e == 0 else None
+    for t in (ta, tb, tc, tw):
+        if t is not None:
+            t.destruct()
+    return e, got, want, wsb
+
+
+@pytest.mark.parametrize("m,n,k,ak,bk,beta_zero", [
+    (1, 1, 1, 1, 1, True),
+    (130, 70, 33, 1, 1, False),
+    (129, 300, 4100, 0, 1, True),    # k' = 8200: two K chunks
+    (257, 129, 2100, 1, 0, False),
+    (200, 130, 5000, 0, 0, True),
+])
+def test_c4_three_way_split_against_fp64(m, n, k, ak, bk, beta_zero):
+    """C4 under fast math: one realified real product (2m x 2k and n x 2k
+    split planes) with the complex alpha applied in the FP64 reduction,
+    every operand orientation, beta 0/1, against complex128 products of
+    the same complex64 inputs: <= 1e-6 relative."""
+    e, got, want, wsb = _gemm_c4(m, n, k, ak, bk, beta_zero, fast=1, seed=m + k)
+    assert e == 0 and wsb > 0
+    assert rel_err(got, want) <= 1e-6, (m, n, k)
+
+
+def test_c4_native_unchanged_without_fast_math():
+    e, got, want, wsb = _gemm_c4(257, 129, 2100, 1, 0, False, fast=0, seed=5)
+    assert e == 0 and wsb == 0
+    assert rel_err(got, want) <= 1e-5
+
+
+def test_c4_workspace_is_reported():
+    L = engine().lib
+    m, n, k = 300, 200, 5000
+    kp = -(-2 * k // 32) * 32
+    mp = -(-2 * m // 256) * 256
+    np_ = -(-n // 128) * 128
+    nz = -(-kp // 8192)
+    assert L.talshAmdGemmWorkspace(T.C4, 1, m, n, k) == 3 * (mp + np_) * kp * 2 + nz * 2 * m * n * 4 + 16
