@@ -84,6 +84,7 @@ class GpuBench:
         a = rng.uniform(-1, 1, t.volume).astype(T.NP_DTYPE[t.kind]) if t.kind in (T.R4, T.R8) else \
             (rng.standard_normal(t.volume) + 1j * rng.standard_normal(t.volume)).astype(T.NP_DTYPE[t.kind])
         t.upload(a)
+        return a
 
     def sync(self):
         self.L.talshAmdDeviceSynchronize(0)
@@ -173,16 +174,32 @@ def measured_traffic():
     return None
 
 
+def sampled_error_c3(l, r, d, n=256, seed=5):
+    """Relative error of D(a,b,i,j) = sum_{k,l} L(a,k,i,l) * R(l,b,k,j)
+    (a=b=i=j=96, k=l=128, DLF bodies) at n seeded positions against float64
+    (complex128) sums of the same inputs computed here with numpy."""
+    Lf = l.reshape(96, 128, 96, 128, order="F")
+    Rf = r.reshape(128, 96, 128, 96, order="F")
+    Df = d.reshape(96, 96, 96, 96, order="F")
+    idx = np.random.default_rng(seed).integers(0, 96, (n, 4))
+    ct = np.complex128 if np.iscomplexobj(l) else np.float64
+    want = np.array([np.sum(Lf[a, :, i, :].astype(ct) * Rf[:, b, :, j].astype(ct).T) for a, b, i, j in idx])
+    got = Df[idx[:, 0], idx[:, 1], idx[:, 2], idx[:, 3]].astype(ct)
+    return float(np.linalg.norm(got - want) / np.linalg.norm(want))
+
+
 def run_contractions(gb, reps=3):
     T = gb.T
     out = {}
-    cases = [("c3_r8", T.R8, False), ("c3_r4_fp32", T.R4, False), ("c3_r4_bf16x3", T.R4, True), ("c3_c8", T.C8, False)]
+    cases = [("c3_r8", T.R8, False), ("c3_r4_fp32", T.R4, False), ("c3_r4_bf16x3", T.R4, True), ("c3_c8", T.C8, False),
+             ("c3_c4_fp32", T.C4, False), ("c3_c4_bf16x3", T.C4, True)]
     pat3 = b"D(a,b,i,j)+=L(a,k,i,l)*R(l,b,k,j)"
     for name, kind, fast in cases:
         d = gb.tensor(kind, [96] * 4)
         l = gb.tensor(kind, [96, 128, 96, 128]); r = gb.tensor(kind, [128, 96, 128, 96])
-        gb.fill(l, 1); gb.fill(r, 2)
-        flops = (8.0 if kind == T.C8 else 2.0) * 96 ** 4 * 128 ** 2
+        lh = gb.fill(l, 1); rh = gb.fill(r, 2)
+        cplx = kind in (T.C4, T.C8)
+        flops = (8.0 if cplx else 2.0) * 96 ** 4 * 128 ** 2
         if fast:
             gb.L.talshEnableFastMath(T.DEV_AMD_GPU, 0)
         gb.sync()
@@ -203,10 +220,13 @@ def run_contractions(gb, reps=3):
         peak = FP64_PEAK if kind in (T.R8, T.C8) else (BF16_PEAK if fast else FP32_PEAK)
         tf = flops * reps / (t1 - t0) / 1e12
         gemm_tf = flops * gn / (gms / 1e3) / 1e12 if gms > 0 else None
+        # outside the timed region: the result of the last (non-accumulating) call against float64 sums
+        err = sampled_error_c3(lh, rh, d.download())
         out[name] = dict(tflops=round(tf, 3), gemm_tflops=round(gemm_tf, 3) if gemm_tf else None,
                          gemm_frac_peak=round(gemm_tf / peak, 4) if gemm_tf else None, peak_tflops=peak,
                          ms_per_op=round((t1 - t0) / reps * 1e3, 3), gemm_ms=round(gms / max(gn, 1), 3),
-                         transpose_ms=round(tms / max(gn, 1), 3))
+                         transpose_ms=round(tms / max(gn, 1), 3), rel_err_fp64_sampled=float("%.3g" % err))
+        del lh, rh
         for t in (d, l, r):
             t.destruct()
     # configured CPU-runnable case, extents 32
@@ -344,73 +364,98 @@ def run_selector(gb, n=300, seed=303):
     return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in rep.items()}
 
 
-def cpu_baseline(target_seconds=12.0, contract_seconds=3.0):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(class_seconds=4.0, contract_seconds=3.0):
     """The reference TAL-SH CPU path (oracle/_ref, built from the reference's
     own sources) timed on this host's cores for a bounded sample of the same
-    transposes; falls back to the oracle's C restatement if it is absent.
-    With the reference build present, also times the extent-32 rank-4 R8
-    contraction on DEV_HOST (returned second, None otherwise)."""
+    transposes: each of the four classes (rank-6 extent-24 and rank-4
+    extent-128, R8 and R4) gets about class_seconds, at least one
+    permutation; falls back to the oracle's C restatement if the reference
+    build is absent. With the reference build present, also times the
+    extent-32 rank-4 R8 contraction on DEV_HOST (returned second, None
+    otherwise)."""
     ncores = os.cpu_count() or 1
     os.environ.setdefault("OMP_NUM_THREADS", str(ncores))
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libtalsh_ref.so")
-    perms = perms_rank6()
-    dims = [24] * 6
-    vol = 24 ** 6
-    x = np.random.default_rng(5).uniform(-1, 1, vol)
-    if os.path.exists(ref_so):
-        L = C.CDLL(ref_so)
+    classes = [("rank6_24_r8", 8, [24] * 6, perms_rank6()), ("rank6_24_r4", 4, [24] * 6, perms_rank6()),
+               ("rank4_128_r8", 8, [128] * 4, perms_rank4()), ("rank4_128_r4", 4, [128] * 4, perms_rank4())]
 
-        class Tens(C.Structure):
-            _fields_ = [("a", C.c_void_p), ("b", C.c_void_p), ("c", C.c_void_p), ("d", C.c_void_p),
-                        ("e", C.c_int), ("f", C.c_int)]
+    class Tens(C.Structure):
+        _fields_ = [("a", C.c_void_p), ("b", C.c_void_p), ("c", C.c_void_p), ("d", C.c_void_p),
+                    ("e", C.c_int), ("f", C.c_int)]
+    kind = "reference" if os.path.exists(ref_so) else "port"
+    if kind == "reference":
+        L = C.CDLL(ref_so)
         sz = C.c_size_t(1 << 24); mx = C.c_int()
         L.talshInit(C.byref(sz), C.byref(mx), 0, None, 0, None, 0, None)
         ierr = C.c_int()
         L.talshSetMemAllocPolicyHost(0, 1, C.byref(ierr))
-        dimsc = (C.c_int * 6)(*dims)
         L.talshTensorConstruct.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                            C.c_void_p, C.c_double, C.c_double]
         L.talshTensorCopy.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
         L.talshTensorGetBodyAccess.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int]
-        src, dst = Tens(), Tens()
-        for t in (src, dst):
-            L.talshTensorClean(C.byref(t))
-            assert L.talshTensorConstruct(C.byref(t), 8, 6, dimsc, 0, None, -1, None, 0.0, 0.0) == 0
-        p = C.c_void_p()
-        L.talshTensorGetBodyAccess(C.byref(src), C.byref(p), 8, 0, 0)
-        C.memmove(p, x.ctypes.data, x.nbytes)
-
-        def one(perm):
-            return L.talshTensorCopy(copy_pattern(perm), C.byref(dst), C.byref(src), 0, 0, 6, None)
-        kind = "reference"
     else:
         lib = C.CDLL(os.path.join(ROOT, "oracle", "libcptal.so"))
-        out = np.empty_like(x)
-        dimsc = (C.c_int * 6)(*dims)
+    p = C.c_void_p()
+    total_bytes, total_s, per_class, parts = 0, 0.0, {}, []
+    for label, es, dims, perms in classes:
+        rank = len(dims)
+        vol = int(np.prod(dims))
+        dk = 8 if es == 8 else 4  # TAL-SH data kinds R8 = 8, R4 = 4
+        x = np.random.default_rng(5).uniform(-1, 1, vol).astype(np.float64 if es == 8 else np.float32)
+        dimsc = (C.c_int * rank)(*dims)
+        if kind == "reference":
+            src, dst = Tens(), Tens()
+            for t in (src, dst):
+                L.talshTensorClean(C.byref(t))
+                assert L.talshTensorConstruct(C.byref(t), dk, rank, dimsc, 0, None, -1, None, 0.0, 0.0) == 0
+            L.talshTensorGetBodyAccess(C.byref(src), C.byref(p), dk, 0, 0)
+            C.memmove(p, x.ctypes.data, x.nbytes)
 
-        def one(perm):
-            o2n = (C.c_int * 7)(1, *[q + 1 for q in perm])
-            return lib.cptal_copy_dlf(8, 6, dimsc, o2n, C.c_void_p(x.ctypes.data), C.c_void_p(out.ctypes.data), 0)
-        kind = "port"
-    # warm-up, then as many permutations as fit the time budget
-    one(perms[0])
-    t0 = time.perf_counter()
-    n = 0
-    total = 0
-    while n < len(perms) and time.perf_counter() - t0 < target_seconds:
-        e = one(perms[n])
-        if e:
-            raise RuntimeError(f"CPU baseline copy failed: {e}")
-        total += 2 * vol * 8
-        n += 1
-    dt = time.perf_counter() - t0
-    base = dict(value=round(total / 1e9 / dt, 3), unit="GB/s", cores=int(os.environ["OMP_NUM_THREADS"]),
-                kind=kind, sample=f"{n} of the 64 rank-6 extent-24 R8 permutations via talshTensorCopy on DEV_HOST "
-                                  f"({dt:.1f} s)")
+            def one(perm):
+                return L.talshTensorCopy(copy_pattern(perm), C.byref(dst), C.byref(src), 0, 0, 6, None)
+        else:
+            out = np.empty_like(x)
+
+            def one(perm):
+                o2n = (C.c_int * (rank + 1))(1, *[q + 1 for q in perm])
+                return lib.cptal_copy_dlf(dk, rank, dimsc, o2n, C.c_void_p(x.ctypes.data),
+                                          C.c_void_p(out.ctypes.data), 0)
+        one(perms[0])  # warm-up
+        t0 = time.perf_counter()
+        n = 0
+        while n < len(perms) and (n == 0 or time.perf_counter() - t0 < class_seconds):
+            e = one(perms[n])
+            if e:
+                raise RuntimeError(f"CPU baseline copy failed: {e}")
+            n += 1
+        dt = time.perf_counter() - t0
+        nbytes = 2 * vol * es * n
+        total_bytes += nbytes
+        total_s += dt
+        per_class[label] = round(nbytes / 1e9 / dt, 3)
+        parts.append(f"{n}/{len(perms)} {label}")
+        if kind == "reference":
+            for t in (src, dst):
+                L.talshTensorDestruct(C.byref(t))
+        del x
+    cores = int(os.environ["OMP_NUM_THREADS"])
+    base = dict(value=round(total_bytes / 1e9 / total_s, 3), unit="GB/s", cores=cores, kind=kind,
+                cpu_model=cpu_model(), per_class_gbs=per_class,
+                sample="permutations via talshTensorCopy on DEV_HOST, about %.0f s per class: %s (%.1f s)"
+                       % (class_seconds, ", ".join(parts), total_s))
     c1 = None
     if kind == "reference":
-        for t in (src, dst):
-            L.talshTensorDestruct(C.byref(t))
         # D(a,b,c,d)+=L(d,i,b,j)*R(j,c,i,a), extents 32, on the reference's CPU path
         L.talshTensorContract.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
                                           C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]

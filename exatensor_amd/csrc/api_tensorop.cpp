@@ -59,8 +59,11 @@ int talshTensorOpSetArgument(talsh_tens_op_t *op, const talsh_tens_t *tensor, co
 int talshTensorOpSpecify(talsh_tens_op_t *op, int operation_kind, int data_kind, const char *symbolic_pattern,
                          double prefactor_real, double prefactor_imag)
 {
+  // the pattern string is referenced, not copied: the caller keeps it alive
+  // for the life of the operation (as in TALSH/talshc.cpp:2665-2689)
   if (!op) return TALSH_INVALID_ARGS;
   if (tens_valid_data_kind(data_kind, nullptr) != YEP) return TALSH_INVALID_ARGS;
+  if (op->opkind != TALSH_TENSOR_NOOP) return TALSH_NOT_ALLOWED;  // specified once
   op->opkind = operation_kind;
   op->data_kind = data_kind;
   op->symb_pattern = symbolic_pattern;
@@ -72,7 +75,8 @@ int talshTensorOpSpecify(talsh_tens_op_t *op, int operation_kind, int data_kind,
 
 int talshTensorOpSetExecDevice(talsh_tens_op_t *op, int dev_id, int dev_kind)
 {
-  if (!op) return TALSH_INVALID_ARGS;
+  if (!op || dev_id < 0) return TALSH_INVALID_ARGS;
+  if (op->stage != TALSH_OP_DEFINED) return TALSH_NOT_ALLOWED;  // after Specify, before activation
   op->exec_dev_id = dev_kind == DEV_DEFAULT ? dev_id : talshFlatDevId(dev_kind, dev_id);
   return 0;
 }

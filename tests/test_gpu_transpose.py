@@ -191,3 +191,49 @@ def test_wide_offsets_and_unfusable_runs():
     j0, j1 = t % 65536, t // 65536
     want = ((j0 + 65536 * (j1 + 32769 * j2)) & 0xFFFFFFFF).astype(np.uint32)
     assert np.array_equal(got[o.astype(np.int64)], want)
+
+
+@pytest.mark.parametrize("rank,ext", [(6, 24), (4, 128)])
+@pytest.mark.parametrize("kind", [T.R4, T.R8])
+def test_configured_workload_every_permutation(rank, ext, kind):
+    """Every transpose of the benchmark workload at full size: the 64 seeded
+    non-identity rank-6 permutations (extent 24) or all 23 non-identity
+    rank-4 permutations (extent 128), in R4 and R8. Every element carries
+    its own input index: integer-valued doubles in R8, the index as the bit
+    pattern in R4. For each permutation 1M random output positions plus the
+    first and last 64K are checked bit for bit against the permutation's
+    index formula."""
+    import bench
+    from gpu_util import pinned
+    eng = engine()
+    L = eng.lib
+    perms = bench.perms_rank6() if rank == 6 else bench.perms_rank4()
+    assert len(perms) == (64 if rank == 6 else 23)
+    dims = [ext] * rank
+    n = ext ** rank
+    if kind == T.R8:
+        x = np.arange(n, dtype=np.float64)
+    else:
+        x = np.arange(n, dtype=np.uint32).view(np.float32)
+    tl = gpu_tensor(x, dims)
+    del x
+    td = T.Tensor(L, kind, dims, eng.gpu(0))
+    assert td.err == 0
+    out = pinned(n * (8 if kind == T.R8 else 4), np.float64 if kind == T.R8 else np.float32)
+    rng = np.random.default_rng(1000 * rank + kind)
+    edge = 1 << 16
+    for perm in perms:
+        e = L.talshTensorCopy(bench.copy_pattern(perm), td.ptr, tl.ptr, eng.gpu(0), T.DEV_NULL, T.COPY_MT, None)
+        assert e == 0, (perm, e)
+        td.download(out)
+        o = np.concatenate([rng.integers(0, n, 1 << 20, dtype=np.int64), np.arange(edge, dtype=np.int64),
+                            np.arange(n - edge, n, dtype=np.int64)])
+        # output multi-index j (dimension 0 fastest); input dimension k sits at output position perm[k]
+        j = [(o // ext ** p) % ext for p in range(rank)]
+        want = sum(j[perm[k]] * ext ** k for k in range(rank))
+        got = out[o]
+        if kind == T.R8:
+            assert np.array_equal(got, want.astype(np.float64)), perm
+        else:
+            assert np.array_equal(got.view(np.uint32), want.astype(np.uint32)), perm
+    td.destruct(); tl.destruct()

@@ -329,3 +329,81 @@ def test_random_contractions_vs_live_reference(oracle, reference):
         if e1 == 0:
             assert _rel(d, out) <= 1e-12, p
         done += 1
+
+
+def _einsum_conj(pattern, D, dd, L, ld, R, rd, alpha, accumulative=True):
+    """numpy restatement of the complex contraction with conjugation flags:
+    L+ / R+ conjugate that operand; D+ conjugates both operands and not alpha
+    (TALSH/tensor_algebra_cpu.F90:3827-3838). Computed in complex128 from
+    the given (possibly complex64) values; DLF bodies are Fortran order."""
+    import re
+    m = re.fullmatch(r"D(\+?)\(([^)]*)\)\+=L(\+?)\(([^)]*)\)\*R(\+?)\(([^)]*)\)", pattern.replace(" ", ""))
+    dc, dl, lc, ll, rc, rl = m.groups()
+    lab = lambda s: s.split(",") if s else []
+    dl, ll, rl = lab(dl), lab(ll), lab(rl)
+    conj_l = bool(lc) != bool(dc)
+    conj_r = bool(rc) != bool(dc)
+    Lf = L.astype(np.complex128).reshape(ld, order="F")
+    Rf = R.astype(np.complex128).reshape(rd, order="F")
+    if conj_l:
+        Lf = np.conj(Lf)
+    if conj_r:
+        Rf = np.conj(Rf)
+    letters = {x: chr(ord("a") + i) for i, x in enumerate(dict.fromkeys(dl + ll + rl))}
+    sub = "".join(letters[x] for x in ll) + "," + "".join(letters[x] for x in rl) + "->" + \
+        "".join(letters[x] for x in dl)
+    prod = alpha * np.einsum(sub, Lf, Rf)
+    base = D.astype(np.complex128).reshape(dd, order="F") if accumulative else 0
+    return (base + prod).ravel(order="F")
+
+
+@pytest.mark.parametrize("kind", [C8, C4])
+def test_conjugated_complex_contractions_vs_einsum(oracle, kind):
+    """The conjugated complex forms, where the reference's -DNO_BLAS build
+    cannot pin the oracle (R+ operands): the oracle's contraction and its
+    long-double evaluator against numpy.einsum with np.conj, for R+, L+R+,
+    D+ (which flips both operand flags) and mixed forms, accumulating and
+    not, complex alpha. 1e-12 (C8), 1e-5 (C4, computed in complex64 by the
+    oracle) relative Frobenius error."""
+    rng = np.random.default_rng(77 + kind)
+    pats = ["D(a,b)+=L(a,i)*R+(i,b)", "D(a,b)+=L+(a,i)*R+(i,b)", "D+(a,b)+=L(i,a)*R(b,i)",
+            "D+(a,b)+=L(a,i)*R+(i,b)", "D+(a,b,c)+=L+(c,i,a)*R(b,i)", "D(a,b,c,d)+=L(d,i,b,j)*R+(j,c,i,a)",
+            "D+(a,b,c,d)+=L+(d,i,b,j)*R+(j,c,i,a)", "D(a,b,i,j)+=L+(a,k,i,l)*R(l,b,k,j)",
+            "D(x,y)+=L(x,q,p)*R+(p,y,q)", "D(a,b)+=L(a)*R+(b)"]
+    dt = NP_DTYPE[kind]
+    for p in pats:
+        e, dig, dr, lr, rr, cj = oracle.str2dig(p)
+        assert e == 0, p
+        import re
+        m = re.fullmatch(r"D\+?\(([^)]*)\)\+=L\+?\(([^)]*)\)\*R\+?\(([^)]*)\)", p)
+        dl, ll, rl = (g.split(",") for g in m.groups())
+        ext = {x: int(rng.integers(2, 7)) for x in set(dl + ll + rl)}
+        dd = [ext[x] for x in dl]; ld = [ext[x] for x in ll]; rd = [ext[x] for x in rl]
+        mk = lambda n: (rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)).astype(dt)
+        D, L, R = mk(int(np.prod(dd))), mk(int(np.prod(ld))), mk(int(np.prod(rd)))
+        alpha = complex(0.6, -0.35)
+        tol = 1e-12 if kind == C8 else 1e-5
+        for acc in (True, False):
+            want = _einsum_conj(p, D, dd, L, ld, R, rd, alpha, acc)
+            d = D.copy()
+            assert oracle.contract(p, d, dd, L, ld, R, rd, alpha, acc) == 0
+            assert _rel(d, want) <= tol, (p, acc)
+            ex = oracle.contract_exact(p, D, dd, L, ld, R, rd, alpha, acc)
+            assert _rel(ex, want) <= 1e-12, (p, acc)
+
+
+def test_einsum_conj_restatement_detects_a_wrong_flag(oracle):
+    """The einsum restatement is sensitive to each flag: dropping R's
+    conjugation, or conjugating alpha under D+, changes the result well
+    beyond the tolerance."""
+    rng = np.random.default_rng(3)
+    mk = lambda n: rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    L, R, D = mk(12), mk(12), mk(9)
+    good = _einsum_conj("D(a,b)+=L(a,i)*R+(i,b)", D, [3, 3], L, [3, 4], R, [4, 3], 0.5 + 0.5j)
+    bad = _einsum_conj("D(a,b)+=L(a,i)*R(i,b)", D, [3, 3], L, [3, 4], R, [4, 3], 0.5 + 0.5j)
+    assert _rel(bad, good) > 1e-3
+    d = D.copy()
+    assert oracle.contract("D+(a,b)+=L(a,i)*R(i,b)", d, [3, 3], L, [3, 4], R, [4, 3], 0.5 + 0.5j) == 0
+    conj_alpha = _einsum_conj("D+(a,b)+=L(a,i)*R(i,b)", D, [3, 3], L, [3, 4], R, [4, 3], 0.5 - 0.5j)
+    assert _rel(d, conj_alpha) > 1e-3
+    assert _rel(d, _einsum_conj("D+(a,b)+=L(a,i)*R(i,b)", D, [3, 3], L, [3, 4], R, [4, 3], 0.5 + 0.5j)) <= 1e-12
