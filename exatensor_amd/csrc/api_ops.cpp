@@ -11,10 +11,15 @@
 // touch the same device image are ordered through per-image events.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "contract_plan.hpp"
+#include "selector.hpp"
 #include "kernels.hpp"
 #include "runtime_impl.hpp"
 #include "talsh_amd.h"
@@ -400,6 +405,101 @@ using namespace xt;
 // plan selector extension.
 static int g_plan_variant = -1;
 static int g_gemm_tile = -1;
+
+// How GEMM-shaped contractions choose their plan (talshAmdSetPlanMode, or
+// TALSH_PLAN_MODE = heuristic | learned | autotune at first use):
+//   0 heuristic: fewest transpose bytes, default tile;
+//   1 learned (default): the learned selector where one is installed and the
+//     contraction lies in its training domain; outside that domain (with a
+//     model installed) the exhaustive autotuner; no model: the heuristic;
+//   2 autotune: the exhaustive autotuner for every shape.
+// The autotuner's choices are memoized per shape (pattern, extents, kind,
+// precision mode, accumulation).
+static int g_plan_mode = -1;
+static std::mutex g_tune_mu;
+static std::unordered_map<std::string, std::pair<int, int>> g_tune_memo;
+
+static int plan_mode()
+{
+  if (g_plan_mode < 0) {
+    const char *e = std::getenv("TALSH_PLAN_MODE");
+    g_plan_mode = 1;
+    if (e && (!std::strcmp(e, "autotune") || !std::strcmp(e, "2"))) g_plan_mode = 2;
+    if (e && (!std::strcmp(e, "heuristic") || !std::strcmp(e, "0"))) g_plan_mode = 0;
+  }
+  return g_plan_mode;
+}
+
+static int contract_impl(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens, double sr,
+                         double si, int dev_id, int dev_kind, int copy_ctrl, int accumulative, talsh_task_t *task,
+                         int force_variant, int force_tile);
+
+namespace {
+
+bool resident_on(const talsh_tens_t *t, int flat)
+{
+  for (int i = 0; i < t->ndev; ++i)
+    if (t->dev_rsc[i].dev_id == flat && t->avail[i] == YEP) return true;
+  return false;
+}
+
+// Exhaustive autotuner: every (matricization, tile) candidate of the
+// contraction runs once as a warm-up and once or twice timed (kernel time of
+// its transposes and matrix product, HIP events on the launch stream) on the
+// caller's L and R images into a scratch D on the executing GPU; the fastest
+// is memoized under key. Blocking. false when it cannot tune now (L or R not
+// resident on the GPU, no room for the scratch D, no candidate ran), leaving
+// the choice to the heuristic for this call.
+bool autotune_choose(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens, double sr,
+                     double si, Device *d, bool acc, const std::vector<int> &dig, int dr, int lr, int rr,
+                     const std::vector<int> &dd, const std::vector<int> &ld, const std::vector<int> &rd, int kind,
+                     const std::string &key, int *variant, int *tile)
+{
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune_memo.find(key);
+    if (it != g_tune_memo.end()) {
+      *variant = it->second.first;
+      *tile = it->second.second;
+      return true;
+    }
+  }
+  if (!resident_on(ltens, d->flat_id) || !resident_on(rtens, d->flat_id)) return false;
+  std::vector<PlanCandidate> cands;
+  if (plan_candidates(dig, dr, lr, rr, dd, ld, rd, kind, d->r4mode, acc, d->num_cus, cands) || cands.empty())
+    return false;
+  talsh_tens_t scratch;
+  talshTensorClean(&scratch);
+  if (talshTensorConstruct(&scratch, kind, dr, dd.data(), d->flat_id, nullptr, -1, nullptr, 0.0, 0.0)) return false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> sink;
+  double best = -1.0;
+  int bv = -1, bt = -1;
+  for (const PlanCandidate &c : cands) {
+    double tmin = -1.0;
+    for (int rep = 0; rep < 3; ++rep) {
+      prof_set_sink(&sink);
+      const int e = contract_impl(cptrn, &scratch, ltens, rtens, sr, si, d->flat_id, DEV_DEFAULT, COPY_MTT,
+                                  acc ? YEP : NOPE, nullptr, c.variant, c.tile);
+      prof_set_sink(nullptr);
+      const double ms = prof_sink_drain(sink);
+      if (e || ms < 0) { tmin = -1.0; break; }
+      if (rep == 0 && ms < 20.0) continue;  // warm-up (long candidates: the first run counts)
+      if (tmin < 0 || ms < tmin) tmin = ms;
+      if (ms >= 20.0) break;
+    }
+    if (tmin >= 0 && (best < 0 || tmin < best)) { best = tmin; bv = c.variant; bt = c.tile; }
+  }
+  talshTensorDestruct(&scratch);
+  if (bv < 0) return false;
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  if (g_tune_memo.size() >= 65536) g_tune_memo.clear();
+  g_tune_memo[key] = {bv, bt};
+  *variant = bv;
+  *tile = bt;
+  return true;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -891,6 +991,18 @@ int talshTensorAdd_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens,
 int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens, double sr,
                         double si, int dev_id, int dev_kind, int copy_ctrl, int accumulative, talsh_task_t *task)
 {
+  return contract_impl(cptrn, dtens, ltens, rtens, sr, si, dev_id, dev_kind, copy_ctrl, accumulative, task, -1, -1);
+}
+
+}  // extern "C"
+
+// force_variant >= 0: plan with that matricization and GEMM tile (the
+// autotuner's candidate runs); otherwise as talshAmdSetPlanVariant and the
+// plan mode decide.
+static int contract_impl(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens, double sr,
+                         double si, int dev_id, int dev_kind, int copy_ctrl, int accumulative, talsh_task_t *task,
+                         int force_variant, int force_tile)
+{
   int err = 0;
   Device *d = exec_device(dev_id, dev_kind, &err);
   if (!d) return err;
@@ -931,13 +1043,24 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
   args[2].t = rtens;
   ContractPlan P;
   int plan_variant = g_plan_variant, plan_tile = g_gemm_tile;
+  if (force_variant >= 0) {
+    plan_variant = force_variant;
+    plan_tile = force_tile;
+  }
   if (dr == 0 && lr == 0 && rr == 0) P.kase = 0;
   else if ((lr == 0) != (rr == 0)) P.kase = 1;
   else if (dr == 0) P.kase = 2;
   else {
-    if (plan_variant < 0) {  // learned selector, when one is installed
-      int v, t;
-      if (selector_choose(dig, dr, lr, rr, dd, ld, rd, kind, d->r4mode, acc, d->num_cus, &v, &t)) {
+    if (plan_variant < 0) {
+      const int mode = plan_mode();
+      int v = -1, t = -1;
+      bool chosen = mode == 1 && selector_choose(dig, dr, lr, rr, dd, ld, rd, kind, d->r4mode, acc, d->num_cus, &v, &t);
+      if (!chosen && (mode == 2 || (mode == 1 && selector_installed()))) {
+        const std::string key = plan_key(dig, lr, rr, dd, ld, rd, kind, d->r4mode, acc, d->num_cus);
+        chosen = autotune_choose(cptrn, dtens, ltens, rtens, sr, si, d, acc, dig, dr, lr, rr, dd, ld, rd, kind, key,
+                                 &v, &t);
+      }
+      if (chosen) {
         plan_variant = v;
         if (plan_tile < 0) plan_tile = t;
       }
@@ -1086,6 +1209,44 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
   if (!cplx) body.aim = si;  // real kinds: GEMM paths use Re(alpha); the scaled-add case needs the full factor
   if (!cplx && P.kase != 1) body.aim = 0.0;
   return run_task(task, d, (unsigned)copy_ctrl, args, kind, flops, body);
+}
+
+extern "C" {
+
+int talshAmdSetPlanMode(int mode)
+{
+  if (mode < 0 || mode > 2) return TALSH_INVALID_ARGS;
+  g_plan_mode = mode;
+  return 0;
+}
+
+int talshAmdGetPlanMode(void) { return plan_mode(); }
+
+int talshAmdAutotuneChoice(const char *cptrn, int drank, const int *ddims, int lrank, const int *ldims, int rrank,
+                           const int *rdims, int data_kind, int accumulative, int fast_math, int *variant, int *tile)
+{
+  if (!cptrn || !variant || !tile) return TALSH_INVALID_ARGS;
+  int digc[2 * MAX_TENSOR_RANK], dr, lr, rr, cj;
+  if (talsh_get_contr_ptrn_str2dig(cptrn, digc, &dr, &lr, &rr, &cj)) return TALSH_INVALID_ARGS;
+  if (dr != drank || lr != lrank || rr != rrank) return TALSH_INVALID_ARGS;
+  std::vector<int> dig(digc, digc + lr + rr), dd(ddims, ddims + dr), ld(ldims, ldims + lr), rd(rdims, rdims + rr);
+  Engine &E = engine();
+  const int cus = (E.on && !E.gpus.empty()) ? E.gpus[0]->num_cus : 256;
+  const std::string key = plan_key(dig, lr, rr, dd, ld, rd, data_kind,
+                                   fast_math ? R4Mode::kBF16x3 : R4Mode::kNativeF32, accumulative != NOPE, cus);
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  auto it = g_tune_memo.find(key);
+  if (it == g_tune_memo.end()) return TALSH_NOT_FOUND;
+  *variant = it->second.first;
+  *tile = it->second.second;
+  return 0;
+}
+
+int talshAmdAutotuneClear(void)
+{
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune_memo.clear();
+  return 0;
 }
 
 int talshTensorContract_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens, double sr,

@@ -293,11 +293,34 @@ void image_forget_all()
 struct ProfRec { int id; hipEvent_t a, b; };
 static std::mutex g_prof_mu;
 static std::vector<ProfRec> g_prof;
+// When set on this thread, kernel timings go to the sink instead of the
+// engine's profile (the autotuner times candidates without touching the
+// caller's profiling records).
+static thread_local std::vector<std::pair<hipEvent_t, hipEvent_t>> *tl_prof_sink = nullptr;
+
+void prof_set_sink(std::vector<std::pair<hipEvent_t, hipEvent_t>> *sink) { tl_prof_sink = sink; }
+
+double prof_sink_drain(std::vector<std::pair<hipEvent_t, hipEvent_t>> &sink)
+{
+  double t = 0;
+  bool ok = true;
+  for (auto &r : sink) {
+    float ms = 0;
+    if (hipEventSynchronize(r.second) == hipSuccess && hipEventElapsedTime(&ms, r.first, r.second) == hipSuccess)
+      t += ms;
+    else
+      ok = false;
+    event_put(r.first, true);
+    event_put(r.second, true);
+  }
+  sink.clear();
+  return ok ? t : -1.0;
+}
 
 void prof_begin(hipStream_t s, int id, hipEvent_t *pair)
 {
   pair[0] = pair[1] = nullptr;
-  if (!engine().profiling) return;
+  if (!engine().profiling && !tl_prof_sink) return;
   pair[0] = event_get(true);
   pair[1] = event_get(true);
   if (!pair[0] || !pair[1]) {
@@ -314,6 +337,10 @@ void prof_end(hipStream_t s, int id, hipEvent_t *pair)
 {
   if (!pair[0] || !pair[1]) return;
   (void)hipEventRecord(pair[1], s);
+  if (tl_prof_sink) {
+    tl_prof_sink->emplace_back(pair[0], pair[1]);
+    return;
+  }
   std::lock_guard<std::mutex> g(g_prof_mu);
   g_prof.push_back(ProfRec{id, pair[0], pair[1]});
 }

@@ -2,6 +2,7 @@
 // contraction, their features, and the learned plan selector.
 #pragma once
 
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -52,6 +53,13 @@ int plan_candidates(const std::vector<int> &dig, int dr, int lr, int rr, const s
 // heuristic.
 constexpr double kSelectorMaxLog2Flops = 35.5;
 constexpr double kSelectorMaxLog2Volume = 26.5;
+
+// Memoization key of a plan choice: everything the candidates and their
+// timings depend on (pattern, extents, data kind, precision mode,
+// accumulation, compute units).
+std::string plan_key(const std::vector<int> &dig, int lr, int rr, const std::vector<int> &dd,
+                     const std::vector<int> &ld, const std::vector<int> &rd, int kind, R4Mode r4mode, bool acc,
+                     int num_cus);
 
 // If a selector model is installed: its choice of (variant, tile) for this
 // contraction (memoized per shape) and true; false otherwise (no model, or

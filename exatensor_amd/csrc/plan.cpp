@@ -279,6 +279,24 @@ float MlpModel::predict_host(const float *x) const
   return mlp_forward_host(s, params.data(), xn) * ystd + ymean;
 }
 
+std::string plan_key(const std::vector<int> &dig, int lr, int rr, const std::vector<int> &dd,
+                     const std::vector<int> &ld, const std::vector<int> &rd, int kind, R4Mode r4mode, bool acc,
+                     int num_cus)
+{
+  std::string key;
+  key.reserve(64);
+  auto put = [&](int v) { key.append(std::to_string(v)).push_back(','); };
+  put(kind); put((int)r4mode); put(acc); put(lr); put(rr); put(num_cus);
+  for (int v : dig) put(v);
+  key.push_back('|');
+  for (int v : dd) put(v);
+  key.push_back('|');
+  for (int v : ld) put(v);
+  key.push_back('|');
+  for (int v : rd) put(v);
+  return key;
+}
+
 namespace {
 std::mutex g_sel_mu;
 MlpModel *g_sel = nullptr;
@@ -305,7 +323,8 @@ bool selector_choose(const std::vector<int> &dig, int dr, int lr, int rr, const 
 {
   std::lock_guard<std::mutex> g(g_sel_mu);
   if (!g_sel) return false;
-  {  // outside the model's training domain the heuristic plans
+  if (kind != R8 && kind != R4 && kind != C8) return false;  // the data kinds it was trained on
+  {  // outside the model's training domain: no choice
     double fl = 2.0, vl = 1, vr = 1, vd = 1;  // real flop count, as the domain is defined
     for (size_t i = 0; i < ld.size(); ++i) { fl *= ld[i]; vl *= ld[i]; }
     for (size_t j = 0; j < rd.size(); ++j) {
@@ -316,14 +335,7 @@ bool selector_choose(const std::vector<int> &dig, int dr, int lr, int rr, const 
     if (std::log2(fl) > kSelectorMaxLog2Flops || std::log2(std::max(vl, std::max(vr, vd))) > kSelectorMaxLog2Volume)
       return false;
   }
-  std::string key;
-  key.reserve(64);
-  auto put = [&](int v) { key.append(std::to_string(v)).push_back(','); };
-  put(kind); put((int)r4mode); put(acc); put(lr); put(rr); put(num_cus);
-  for (int v : dig) put(v);
-  for (int v : dd) put(v);
-  for (int v : ld) put(v);
-  for (int v : rd) put(v);
+  const std::string key = plan_key(dig, lr, rr, dd, ld, rd, kind, r4mode, acc, num_cus);
   auto it = g_sel_memo.find(key);
   if (it != g_sel_memo.end()) {
     *variant = it->second.first;

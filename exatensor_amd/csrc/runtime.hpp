@@ -94,6 +94,11 @@ void event_pool_clear();
 
 // ---------------------------------------------------------------------------
 // Profiling of the dominant kernels (event pairs on the launch stream).
+// Redirects this thread's kernel timings into sink (nullptr: back to the
+// engine's profile); prof_sink_drain returns their summed milliseconds (-1 if
+// an event failed) and returns the events to the pool.
+void prof_set_sink(std::vector<std::pair<hipEvent_t, hipEvent_t>> *sink);
+double prof_sink_drain(std::vector<std::pair<hipEvent_t, hipEvent_t>> &sink);
 void prof_begin(hipStream_t s, int kernel_id, hipEvent_t *pair);
 void prof_end(hipStream_t s, int kernel_id, hipEvent_t *pair);
 enum KernelId { kKTranspose = 0, kKGemm = 1, kKGrouped = 2, kKElementwise = 3, kKNum = 4 };

@@ -24,6 +24,8 @@ TALSH_OBJECT_NOT_EMPTY = 1000004
 TALSH_OBJECT_IS_EMPTY = 1000005
 TALSH_IN_PROGRESS = 1000006
 TALSH_NOT_FOUND = 1000009
+TALSH_LIMIT_EXCEEDED = 1000008
+TALSH_NOT_ALLOWED = 1000007
 TALSH_TASK_ERROR = 1999999
 TALSH_TASK_EMPTY = 2000000
 TALSH_TASK_SCHEDULED = 2000001
@@ -124,6 +126,11 @@ def load_library(path=LIB_PATH):
         "talshAmdProfile": (C.c_int, [C.c_int]),
         "talshAmdProfileQuery": (C.c_int, [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
         "talshAmdSetPlanVariant": (C.c_int, [C.c_int, C.c_int]),
+        "talshAmdSetPlanMode": (C.c_int, [C.c_int]),
+        "talshAmdGetPlanMode": (C.c_int, []),
+        "talshAmdAutotuneChoice": (C.c_int, [C.c_char_p, C.c_int, _IP, C.c_int, _IP, C.c_int, _IP, C.c_int, C.c_int,
+                                             C.c_int, _IP, _IP]),
+        "talshAmdAutotuneClear": (C.c_int, []),
         "talshAmdPlanCandidates": (C.c_int, [C.c_char_p, C.c_int, _IP, C.c_int, _IP, C.c_int, _IP, C.c_int, C.c_int,
                                              C.c_int, C.POINTER(C.c_double), _IP]),
         "talshAmdDeviceSynchronize": (C.c_int, [C.c_int]),

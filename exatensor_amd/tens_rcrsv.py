@@ -1,6 +1,8 @@
 """Hierarchical block-sparse tensors and the contraction split that turns one
 tensor contraction into the list of block (subtensor) contractions the
-batched GPU path executes.
+batched GPU path executes: a Python view of the engine's C-ABI
+(talshAmdHSpace*, talshAmdContractionSplit, talshAmdBatchCreateFromSplit in
+include/talsh_amd.h, implemented in exatensor_amd/csrc/hsplit.cpp).
 
 Follows ExaTENSOR's recursive tensors:
   * HSpace: a vector space of dimension N with its subspace aggregation tree
@@ -10,89 +12,105 @@ Follows ExaTENSOR's recursive tensors:
     being cut into min(length, branching factor) contiguous pieces by the
     unaligned rule of SegIntSplit (:866-979): pieces of length len // m, the
     first len % m of them one longer; pieces longer than one get the next
-    free id, single vectors keep theirs.
+    free id, single vectors keep theirs. The tree is built by the engine;
+    this class mirrors its ranges, parents, levels and children.
   * RcrsvTensor: a tensor whose every dimension is a subspace of an HSpace,
     composed of subtensors (one child subspace per dimension; by default
     the full Cartesian product of the children, dimension 0 fastest, and a
     caller may keep only some of them: block sparsity).
   * contraction_split: TensContractionSplitFunc
-    (INTRAVIRT/tensor_recursive.F90:9202-9784) -- align the tree levels of
-    dimensions that the pattern pairs (the finer side is promoted to its
-    ancestor at the coarser level, align_levels :9350-9440, decided on each
-    argument's first subtensor), describe every subtensor by its (promoted)
-    subspace ids, join L and R on the contracted ids, form the Cartesian
-    products, and keep those whose open ids name an existing D subtensor.
-    Output order: D subtensors in ascending order of their id multi-index
-    (dimension 1 most significant), and within one D subtensor ascending
-    contracted multi-index, then R subtensor, then L subtensor -- the order
-    the reference's sorted merge produces.
+    (INTRAVIRT/tensor_recursive.F90:9202-9784), computed by the engine --
+    align the tree levels of dimensions that the pattern pairs (the finer
+    side is promoted to its ancestor at the coarser level, align_levels
+    :9350-9440, decided on each argument's first subtensor), describe every
+    subtensor by its (promoted) subspace ids, join L and R on the contracted
+    ids, form the Cartesian products, and keep those whose open ids name an
+    existing D subtensor. Output order: D subtensors in ascending order of
+    their id multi-index (dimension 1 most significant), and within one D
+    subtensor ascending contracted multi-index, then R subtensor, then L
+    subtensor -- the order the reference's sorted merge produces.
 Every subtensor of an argument must sit at one tree level per dimension
 (the reference decides the alignment on the first subtensor and assumes
 this; here a mix raises ValueError). Symmetric-dimension restrictions
 (get_dim_symmetry) are not supported: all dimensions are independent.
 
-to_batch() makes engine tensors for the subtensors of D, L and R and a
-talshAmdBatch of the split, i.e. the block-level loop of
+to_batch() makes a talshAmdBatch of the split from engine tensors of the
+subtensors of D, L and R, i.e. the block-level loop of
 INTERVIRT/tavp_worker.F90:9440-9480 as one grouped launch.
 """
+import ctypes as C
 import itertools
 
 from . import talsh as T
 
 CMP_EQ, CMP_LT, CMP_GT, CMP_OV = 0, -1, 1, 2
 
+_IP = C.POINTER(C.c_int)
+_VPP = C.POINTER(C.c_void_p)
+
+
+def _lib():
+    L = T.load_library()
+    if not getattr(L, "_hsplit_bound", False):
+        sig = {
+            "talshAmdHSpaceCreate": (C.c_int, [C.c_int, C.c_int, _IP, C.POINTER(C.c_void_p)]),
+            "talshAmdHSpaceDestroy": (C.c_int, [C.c_void_p]),
+            "talshAmdHSpaceInfo": (C.c_int, [C.c_void_p, _IP, _IP]),
+            "talshAmdHSpaceSubspace": (C.c_int, [C.c_void_p, C.c_int, _IP, _IP, _IP, _IP, _IP]),
+            "talshAmdHSpaceChildren": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _IP]),
+            "talshAmdContractionSplit": (C.c_int, [C.c_char_p, C.c_int, _VPP, C.c_int, _IP, C.c_int, _VPP, C.c_int,
+                                                   _IP, C.c_int, _VPP, C.c_int, _IP, C.c_longlong, _IP,
+                                                   C.POINTER(C.c_longlong)]),
+            "talshAmdBatchCreateFromSplit": (C.c_int, [C.c_char_p, C.c_longlong, _IP, C.POINTER(T._TP), C.c_int,
+                                                       C.POINTER(T._TP), C.c_int, C.POINTER(T._TP), C.c_int,
+                                                       C.c_double, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        L._hsplit_bound = True
+    return L
+
 
 class HSpace:
-    """Subspace aggregation tree of an N-dimensional space (see module doc)."""
+    """Subspace aggregation tree of an N-dimensional space (see module doc),
+    built by the engine (talshAmdHSpaceCreate)."""
 
     def __init__(self, dim, branch_fac=()):
         if dim <= 0:
             raise ValueError("space dimension must be positive")
         if any(b < 2 for b in branch_fac):
             raise ValueError("branching factors must be >= 2")
+        L = _lib()
+        self._L = L
+        self.handle = C.c_void_p()
+        bf = T.ints(branch_fac)
+        e = L.talshAmdHSpaceCreate(dim, len(branch_fac), bf, C.byref(self.handle))
+        if e:
+            raise ValueError(f"talshAmdHSpaceCreate: {e}")
+        n, root = C.c_int(), C.c_int()
+        L.talshAmdHSpaceInfo(self.handle, C.byref(n), C.byref(root))
         self.dim = dim
-        self.lo = list(range(dim))
-        self.hi = [i + 1 for i in range(dim)]
-        self.parent = [-1] * dim
-        self.kids = [[] for _ in range(dim)]
-        self.lev = [0] * dim
-        if dim == 1:
-            self.root = 0
-            return
-        self.root = self._new(0, dim, -1, 0)
-        level, frontier = 0, [self.root]
-        while frontier:
-            brf = branch_fac[level] if level < len(branch_fac) else 2
-            nxt = []
-            for sid in frontier:
-                length = self.hi[sid] - self.lo[sid]
-                m = min(length, brf)
-                if m <= 1:
-                    continue
-                base, extra = divmod(length, m)
-                lb = self.lo[sid]
-                for p in range(m):
-                    ub = lb + base + (1 if p < extra else 0)
-                    if ub - lb > 1:
-                        child = self._new(lb, ub, sid, level + 1)
-                        nxt.append(child)
-                    else:
-                        child = lb  # the single vector keeps its id
-                        self.parent[child] = sid
-                        self.lev[child] = level + 1
-                    self.kids[sid].append(child)
-                    lb = ub
-            frontier = nxt
-            level += 1
+        self.root = root.value
+        self.lo, self.hi, self.parent, self.lev, self.kids = [], [], [], [], []
+        off, ext, lev, par, nk = (C.c_int() for _ in range(5))
+        for sid in range(n.value):
+            L.talshAmdHSpaceSubspace(self.handle, sid, C.byref(off), C.byref(ext), C.byref(lev), C.byref(par),
+                                     C.byref(nk))
+            kids = (C.c_int * max(nk.value, 1))()
+            L.talshAmdHSpaceChildren(self.handle, sid, nk.value, kids)
+            self.lo.append(off.value)
+            self.hi.append(off.value + ext.value)
+            self.lev.append(lev.value)
+            self.parent.append(par.value)
+            self.kids.append(list(kids[:nk.value]))
 
-    def _new(self, lo, hi, parent, level):
-        sid = len(self.lo)
-        self.lo.append(lo)
-        self.hi.append(hi)
-        self.parent.append(parent)
-        self.kids.append([])
-        self.lev.append(level)
-        return sid
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and h.value:
+            self._L.talshAmdHSpaceDestroy(h)
+            self.handle = C.c_void_p()
 
     @property
     def num_subspaces(self):
@@ -182,76 +200,42 @@ def _pattern_map(pattern, drank, lrank, rrank):
     return dig
 
 
+def _arg(t):
+    spaces = (C.c_void_p * max(t.rank, 1))(*[sp.handle.value for sp in t.spaces])
+    flat = [i for s in t.subtensors for i in s]
+    return spaces, len(t.subtensors), T.ints(flat)
+
+
 def contraction_split(pattern, dten, lten, rten):
     """Block contractions (d, l, r) -- indices into dten/lten/rten.subtensors
-    -- of D += L * R (see module doc for the rule and the order)."""
+    -- of D += L * R (see module doc for the rule and the order), computed by
+    talshAmdContractionSplit."""
     for t in (dten, lten, rten):
         if not t.subtensors:
             raise ValueError(f"tensor {t.name} has no subtensors (call split())")
-    dr, lr, rr = dten.rank, lten.rank, rten.rank
-    cp = _pattern_map(pattern, dr, lr, rr)
-    # the alignment is decided on the first subtensor, so every subtensor of
-    # an argument must sit at one tree level per dimension
+    _pattern_map(pattern, dten.rank, lten.rank, rten.rank)
     for t in (dten, lten, rten):
         for i, sp in enumerate(t.spaces):
             if len({sp.level(s[i]) for s in t.subtensors}) > 1:
                 raise ValueError(f"subtensors of {t.name} mix tree levels in dimension {i}")
-    # level alignment, decided on the first subtensor of each argument
-    adj = [[0] * dr, [0] * lr, [0] * rr]
-    first = (dten.subtensors[0], lten.subtensors[0], rten.subtensors[0])
-    tens = (dten, lten, rten)
-
-    def align(a, i, b, j):
-        la = tens[a].spaces[i].level(first[a][i])
-        lb = tens[b].spaces[j].level(first[b][j])
-        if la < lb:
-            adj[b][j] = lb - la
-        elif la > lb:
-            adj[a][i] = la - lb
-
-    for i in range(lr):
-        j = cp[i]
-        if j < 0:
-            align(1, i, 2, -j - 1)
-        else:
-            align(1, i, 0, j - 1)
-    for i in range(rr):
-        j = cp[lr + i]
-        if j > 0:
-            align(2, i, 0, j - 1)
-
-    def descr(a, ids):
-        return tuple(tens[a].spaces[d].ancestor(s, adj[a][d]) if adj[a][d] else s for d, s in enumerate(ids))
-
-    # dimension orders: contracted dims (L order) then open dims
-    lc = [i for i in range(lr) if cp[i] < 0]
-    rc = [-cp[i] - 1 for i in lc]
-    lo = [i for i in range(lr) if cp[i] > 0]
-    ro = [i for i in range(rr) if cp[lr + i] > 0]
-    # D position of each open dim of L, then of R
-    dpos = [cp[i] - 1 for i in lo] + [cp[lr + i] - 1 for i in ro]
-    ldesc = [descr(1, s) for s in lten.subtensors]
-    rdesc = [descr(2, s) for s in rten.subtensors]
-    ddesc = [descr(0, s) for s in dten.subtensors]
-    lgroups, rgroups = {}, {}
-    for n, d in enumerate(ldesc):
-        lgroups.setdefault(tuple(d[i] for i in lc), []).append(n)
-    for n, d in enumerate(rdesc):
-        rgroups.setdefault(tuple(d[i] for i in rc), []).append(n)
-    products = {}  # D multi-index -> [(contracted key, r, l)]
-    for key in sorted(set(lgroups) & set(rgroups)):
-        for r in sorted(rgroups[key], key=lambda n: (tuple(rdesc[n][i] for i in ro), n)):
-            for l in sorted(lgroups[key], key=lambda n: (tuple(ldesc[n][i] for i in lo), n)):
-                open_ids = [ldesc[l][i] for i in lo] + [rdesc[r][i] for i in ro]
-                dkey = [0] * dr
-                for p, v in zip(dpos, open_ids):
-                    dkey[p] = v
-                products.setdefault(tuple(dkey), []).append((key, r, l))
-    out = []
-    for d in sorted(range(len(ddesc)), key=lambda n: (ddesc[n], n)):
-        for _key, r, l in products.get(ddesc[d], ()):
-            out.append((d, l, r))
-    return out
+    L = _lib()
+    args = []
+    for t in (dten, lten, rten):
+        args.append((t.rank,) + _arg(t))
+    flat = []
+    for rank, spaces, n, sub in args:
+        flat += [rank, spaces, n, sub]
+    n = C.c_longlong()
+    cap = 1 << 12
+    while True:
+        out = (C.c_int * (3 * cap))()
+        e = L.talshAmdContractionSplit(pattern.encode(), *flat, cap, out, C.byref(n))
+        if e == T.TALSH_LIMIT_EXCEEDED and n.value > cap:
+            cap = n.value
+            continue
+        if e:
+            raise ValueError(f"talshAmdContractionSplit: {e}")
+        return [(out[3 * i], out[3 * i + 1], out[3 * i + 2]) for i in range(n.value)]
 
 
 def to_batch(lib, pattern, dten, lten, rten, split, gpu, dbody, lbody, rbody, alpha=1.0, accumulate=True):
@@ -269,15 +253,19 @@ def to_batch(lib, pattern, dten, lten, rten, split, gpu, dbody, lbody, rbody, al
         if not ok:
             raise ValueError("block contraction with subtensors of different tree levels: whole-block batches "
                              "need D, L and R split to matching levels")
+    L = _lib()
+    blocks = []
+    for t, body in ((dten, dbody), (lten, lbody), (rten, rbody)):
+        arr = (T._TP * len(t.subtensors))()
+        for k, ids in enumerate(t.subtensors):
+            arr[k] = C.pointer(body(ids).t)
+        blocks.append(arr)
     n = len(split)
-    arr = [(T._TP * n)() for _ in range(3)]
-    for k, (d, l, r) in enumerate(split):
-        arr[0][k] = C.pointer(dbody(dten.subtensors[d]).t)
-        arr[1][k] = C.pointer(lbody(lten.subtensors[l]).t)
-        arr[2][k] = C.pointer(rbody(rten.subtensors[r]).t)
+    tri = T.ints([x for dlr in split for x in dlr])
     h = C.c_void_p()
-    e = lib.talshAmdBatchCreate(pattern.encode(), n, arr[0], arr[1], arr[2], alpha, T.YEP if accumulate else T.NOPE,
-                                gpu, C.byref(h))
+    e = L.talshAmdBatchCreateFromSplit(pattern.encode(), n, tri, blocks[0], len(dten.subtensors), blocks[1],
+                                       len(lten.subtensors), blocks[2], len(rten.subtensors), alpha,
+                                       T.YEP if accumulate else T.NOPE, gpu, C.byref(h))
     if e != 0:
-        raise RuntimeError(f"talshAmdBatchCreate: {e}")
+        raise RuntimeError(f"talshAmdBatchCreateFromSplit: {e}")
     return h

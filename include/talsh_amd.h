@@ -46,7 +46,8 @@ int talshAmdPermutePlan(int data_kind, int rank, const int *dims, const int *o2n
  * C(m x n, column-major) = alpha * op(A) * op(B) (+ C unless beta_zero);
  * A is k x m k-fastest (a_kfast) or m x k m-fastest, B is k x n k-fastest
  * (b_kfast) or n x k n-fastest. fast_math selects the three-way BF16 split
- * for R4, which needs talshAmdGemmWorkspace() bytes of device workspace; real
+ * for R4 and C4 (C4 realified: one real product over 2m x 2k and n x 2k
+ * split operands), which needs talshAmdGemmWorkspace() bytes of device workspace; real
  * FP64/FP32 products with few output tiles split K when given that workspace
  * (talshAmdGemmWorkspace covers every tile choice). Synchronous; used by the
  * autotuner and the tests. */
@@ -101,9 +102,10 @@ int talshAmdPlanFeatures(const char *cptrn, int drank, const int *ddims, int lra
                          const int *rdims, int data_kind, int accumulative, int fast_math, int cap, int *variants,
                          int *tiles, float *features, int *ncand);
 /* Install a copy of a trained model as the planner of talshTensorContract
- * (NULL removes it; without a model the byte-count heuristic plans, and so it
- * does for contractions beyond the model's training domain of 2^35.5 flops or
- * 2^26.5-element operands). talshInit loads the model file named by the
+ * (NULL removes it; without a model the byte-count heuristic plans; with one,
+ * contractions beyond the model's training domain of 2^35.5 flops,
+ * 2^26.5-element operands and the kinds R8, R4, C8 go to the exhaustive
+ * autotuner, see talshAmdSetPlanMode). talshInit loads the model file named by the
  * environment variable TALSH_PLAN_MODEL, or else plan_model.txt next to
  * libtalsh.so when present; TALSH_PLAN_MODEL set to an empty string disables
  * the selector. */
@@ -112,6 +114,59 @@ int talshAmdSelectorInstall(void *mlp);
  * choice, or the heuristic's (variant 0 with its default GEMM tile). */
 int talshAmdSelectorChoice(const char *cptrn, int drank, const int *ddims, int lrank, const int *ldims, int rrank,
                            const int *rdims, int data_kind, int accumulative, int fast_math, int *variant, int *tile);
+
+/* How GEMM-shaped contractions choose their plan (matricization and GEMM
+ * tile) when no variant is forced with talshAmdSetPlanVariant:
+ *   0 heuristic: fewest transpose bytes, default tile;
+ *   1 learned (default): the installed selector inside its training domain,
+ *     the exhaustive autotuner outside it; with no model installed, the
+ *     heuristic;
+ *   2 autotune: the exhaustive autotuner for every shape.
+ * The autotuner runs every candidate on the caller's L and R (resident on the
+ * executing GPU; otherwise the heuristic plans that call) into a scratch D,
+ * times its transposes and product with HIP events, and memoizes the fastest
+ * per shape (pattern, extents, kind, fast-math mode, accumulation); the first
+ * call of a shape blocks for the search. The initial mode comes from the
+ * environment variable TALSH_PLAN_MODE (heuristic | learned | autotune). */
+int talshAmdSetPlanMode(int mode);
+int talshAmdGetPlanMode(void);
+/* The autotuner's memoized choice for a contraction shape, or
+ * TALSH_NOT_FOUND if it has not tuned that shape; Clear forgets them all. */
+int talshAmdAutotuneChoice(const char *cptrn, int drank, const int *ddims, int lrank, const int *ldims, int rrank,
+                           const int *rdims, int data_kind, int accumulative, int fast_math, int *variant, int *tile);
+int talshAmdAutotuneClear(void);
+
+/* ---- hierarchical block-sparse tensors ----
+ * Subspace aggregation tree of a space of dimension dim (HSpaceCtorSimple,
+ * INTRAVIRT/subspaces.F90:2098-2372): ids 0..dim-1 are the single basis
+ * vectors, dim the full space (the root); level l cuts every aggregate into
+ * min(length, branch_fac[l]) contiguous pieces (2 beyond nlevels), the first
+ * length % m pieces one longer; pieces longer than one get the next id. */
+int talshAmdHSpaceCreate(int dim, int nlevels, const int *branch_fac, void **hspace);
+int talshAmdHSpaceDestroy(void *hspace);
+int talshAmdHSpaceInfo(void *hspace, int *num_subspaces, int *root);
+/* Range [offset, offset + extent) of subspace sid, its tree level, parent
+ * (-1 for the root) and number of children; any output may be NULL. */
+int talshAmdHSpaceSubspace(void *hspace, int sid, int *offset, int *extent, int *level, int *parent, int *nchildren);
+int talshAmdHSpaceChildren(void *hspace, int sid, int cap, int *children);
+/* The contraction split of recursive tensors (TensContractionSplitFunc,
+ * INTRAVIRT/tensor_recursive.F90:9202-9784; its caller is
+ * INTERVIRT/tavp_worker.F90:9467): D, L and R are given by the space of every
+ * dimension (xspaces[rank]) and their subtensors (xsub[n][rank], the subspace
+ * id of each dimension; every subtensor of an argument at one tree level per
+ * dimension). Writes the block contractions as triples (d, l, r) of
+ * subtensor indices into triples[cap][3] and their number into *ntriples;
+ * TALSH_LIMIT_EXCEEDED (with *ntriples set) when cap is too small. Order: D
+ * subtensors by ascending ids (dimension 0 most significant), then contracted
+ * ids, R subtensor, L subtensor. Conjugation flags are rejected. */
+int talshAmdContractionSplit(const char *cptrn, int drank, void *const *dspaces, int nd, const int *dsub, int lrank,
+                             void *const *lspaces, int nl, const int *lsub, int rrank, void *const *rspaces, int nr,
+                             const int *rsub, long long cap, int *triples, long long *ntriples);
+/* talshAmdBatchCreate over the triples of a split: block contraction i is
+ * dblocks[d_i] (+)= alpha * lblocks[l_i] * rblocks[r_i]. */
+int talshAmdBatchCreateFromSplit(const char *cptrn, long long ntriples, const int *triples, talsh_tens_t *const *dblocks,
+                                 int nd, talsh_tens_t *const *lblocks, int nl, talsh_tens_t *const *rblocks, int nr,
+                                 double alpha_re, int accumulative, int dev_num, void **batch);
 
 /* Device bytes one piece of talshTensorContractXL may occupy (slices plus
  * temporaries); contractions above it are decomposed into pieces. 0 restores

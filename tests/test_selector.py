@@ -226,3 +226,103 @@ def test_collect_train_evaluate_pipeline(eng):
     lib.talshAmdSelectorInstall(None)
     assert rep["contractions"] == 12 and rep["mean_regret"] >= 0 and rep["heuristic_mean_regret"] >= 0
     m.close()
+
+
+def _restore_default_model(lib):
+    if os.path.exists(S.DEFAULT_MODEL):
+        m = S.Mlp(lib, path=S.DEFAULT_MODEL)
+        m.install()
+        m.close()
+    else:
+        lib.talshAmdSelectorInstall(None)
+
+
+@pytest.mark.gpu
+def test_model_and_autotune_regret_against_exhaustive(eng):
+    """On 40 seeded contractions outside the training data, every candidate
+    is timed (the exhaustive search). The packaged model's mean regret is
+    below the byte-count heuristic's (what the learned selector gains), and
+    the engine's own exhaustive autotuner (plan mode 2), whose memoized
+    choice is looked up and priced in the same exhaustive table, stays within
+    timing noise of the best: mean regret <= 3%, median 0."""
+    lib = eng.lib
+    if not os.path.exists(S.DEFAULT_MODEL):
+        pytest.skip("no packaged plan model")
+    rng = np.random.default_rng(505)
+    specs, times = [], []
+    for _ in range(40):
+        s = S.random_contraction(rng, log2_vol=(16, 22))
+        _, _, _, tm = S.time_candidates(lib, eng.gpu(0), s)
+        specs.append(s)
+        times.append(tm)
+    m = S.Mlp(lib, path=S.DEFAULT_MODEL)
+    try:
+        rep = S.regret_report(lib, specs, times, m)
+    finally:
+        m.close()
+        _restore_default_model(lib)
+    print("model regret %.4f, heuristic regret %.4f" % (rep["mean_regret"], rep["heuristic_mean_regret"]))
+    assert rep["mean_regret"] < rep["heuristic_mean_regret"]
+    assert lib.talshAmdSetPlanMode(2) == 0
+    lib.talshAmdAutotuneClear()
+    regrets = []
+    try:
+        for s, tm in zip(specs, times):
+            kind = s["kind"]
+            D = T.Tensor(lib, kind, s["ddims"], eng.gpu(0), 0.0)
+            L = T.Tensor(lib, kind, s["ldims"], eng.gpu(0), 0.5)
+            R = T.Tensor(lib, kind, s["rdims"], eng.gpu(0), 0.25)
+            acc = T.YEP if s["acc"] else T.NOPE
+            assert lib.talshTensorContract(s["pattern"].encode(), D.ptr, L.ptr, R.ptr, 1.0, 0.0, eng.gpu(0), T.DEV_NULL,
+                                           T.COPY_MTT, acc, None) == 0
+            for x in (D, L, R):
+                x.destruct()
+            v, t = C.c_int(), C.c_int()
+            assert lib.talshAmdAutotuneChoice(s["pattern"].encode(), len(s["ddims"]), T.ints(s["ddims"]),
+                                              len(s["ldims"]), T.ints(s["ldims"]), len(s["rdims"]),
+                                              T.ints(s["rdims"]), kind, acc, 0, C.byref(v), C.byref(t)) == 0
+            variants, tiles, _ = S.plan_features(lib, s)
+            idx = {(int(a), int(b)): i for i, (a, b) in enumerate(zip(variants, tiles))}
+            regrets.append(tm[idx[(v.value, t.value)]] / tm.min() - 1.0)
+    finally:
+        lib.talshAmdSetPlanMode(1)
+    regrets = np.array(regrets)
+    print("autotune regret mean %.4f median %.4f" % (regrets.mean(), np.median(regrets)))
+    assert regrets.mean() <= 0.03 and np.median(regrets) == 0.0
+
+
+@pytest.mark.gpu
+def test_autotune_fallback_outside_model_domain(eng, oracle):
+    """With the default plan mode and a model installed, a contraction the
+    model was not trained for (C4 data, and a product beyond 2^35.5 flops)
+    is planned by the autotuner: its choice is memoized after the first call,
+    and the result still matches the long-double evaluation."""
+    from gpu_util import contract, rel_err
+    lib = eng.lib
+    if not os.path.exists(S.DEFAULT_MODEL):
+        pytest.skip("no packaged plan model")
+    _restore_default_model(lib)
+    assert lib.talshAmdGetPlanMode() == 1
+    lib.talshAmdAutotuneClear()
+    cases = [("D(a,b,c)+=L(c,i,a)*R(b,i)", T.C4, [40, 33, 27], [27, 50, 40], [33, 50]),
+             ("D(a,b)+=L(i,a)*R(b,i)", T.R8, [4000, 3000], [3000, 4000], [3000, 3000])]
+    for pat, kind, dd, ld, rd in cases:
+        vol = lambda dims: int(np.prod(dims))
+        if kind == T.C4:
+            l, r, d = (oracle.normal(T.C4, vol(x), 60 + i) for i, x in enumerate((ld, rd, dd)))
+        else:
+            l, r, d = (oracle.uniform(T.R8, [vol(x)], 60 + i) for i, x in enumerate((ld, rd, dd)))
+        v, t = C.c_int(), C.c_int()
+        args = (pat.encode(), len(dd), T.ints(dd), len(ld), T.ints(ld), len(rd), T.ints(rd), kind, T.YEP, 0,
+                C.byref(v), C.byref(t))
+        assert lib.talshAmdAutotuneChoice(*args) == T.TALSH_NOT_FOUND
+        e, got = contract(pat, d, dd, l, ld, r, rd, 0.5)
+        assert e == 0
+        assert lib.talshAmdAutotuneChoice(*args) == 0
+        if vol(dd) <= 1 << 17:
+            ex = oracle.contract_exact(pat, d, dd, l, ld, r, rd, 0.5)
+        else:
+            pos = np.random.default_rng(1).choice(vol(dd), 1024, replace=False).astype(np.uint64)
+            ex = oracle.contract_exact(pat, d, dd, l, ld, r, rd, 0.5, True, pos)
+            got = got[pos.astype(np.int64)]
+        assert rel_err(got, ex) <= (1e-5 if kind == T.C4 else 1e-12), pat

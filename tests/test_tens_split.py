@@ -233,3 +233,29 @@ def test_to_batch_rejects_mixed_levels():
     split = TR.contraction_split(PAT, D, L, R)
     with pytest.raises(ValueError):
         TR.to_batch(None, PAT, D, L, R, split, 0, None, None, None)
+
+
+def test_split_c_abi_capacity_and_rejections():
+    """talshAmdContractionSplit directly: a capacity below the number of
+    block contractions reports TALSH_LIMIT_EXCEEDED with the count (and the
+    first cap triples in order), conjugated or rank-mismatched patterns and
+    out-of-range subspace ids are rejected."""
+    import ctypes as C
+    from exatensor_amd import talsh as T
+    sp, D, L, R = _tensors(2, (1, 1, 1), 0.5)
+    full = TR.contraction_split(PAT, D, L, R)
+    assert len(full) > 8
+    lib = TR._lib()
+    args = []
+    for t in (D, L, R):
+        args += [t.rank, *TR._arg(t)]
+    out = (C.c_int * 24)()
+    n = C.c_longlong()
+    assert lib.talshAmdContractionSplit(PAT.encode(), *args, 8, out, C.byref(n)) == T.TALSH_LIMIT_EXCEEDED
+    assert n.value == len(full)
+    assert [tuple(out[3 * i:3 * i + 3]) for i in range(8)] == full[:8]
+    assert lib.talshAmdContractionSplit(b"D(a,b,c,d)+=L(a,k,c,l)*R+(l,b,k,d)", *args, 8, out, C.byref(n)) == \
+        T.TALSH_INVALID_ARGS
+    bad = list(args)
+    bad[7] = T.ints([x if i else 10 ** 6 for i, x in enumerate(sum((list(s) for s in L.subtensors), []))])
+    assert lib.talshAmdContractionSplit(PAT.encode(), *bad, 8, out, C.byref(n)) == T.TALSH_INVALID_ARGS
