@@ -89,9 +89,13 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
     for (int j : ndims_r)
       if (dig[i] == dig[lr + j]) return TALSH_INVALID_ARGS;  // hyper-index
   const int kind = nops > 0 && dtens[0] && dtens[0]->ndev > 0 ? dtens[0]->data_kind[0] : NO_TYPE;
-  if (kind != R8 && kind != R4) return TALSH_NOT_IMPLEMENTED;
-  const int es = kind_size(kind);
-  const bool split3 = kind == R4 && d->r4mode == R4Mode::kBF16x3;
+  if (kind != R8 && kind != R4 && kind != C8 && kind != C4) return TALSH_INVALID_ARGS;
+  const bool cplx = kind == C8 || kind == C4;
+  // conjugation flags of the complex operands (D+ conjugates both)
+  bool lconj = cplx && (cj & 2), rconj = cplx && (cj & 4);
+  if (cplx && (cj & 1)) { lconj = !lconj; rconj = !rconj; }
+  const int es = kind_size(kind) / (cplx ? 2 : 1);  // bytes of one real component
+  const bool split3 = (kind == R4 || kind == C4) && d->r4mode == R4Mode::kBF16x3;
   const int kq = split3 ? 32 : 16;                 // k padding of packed operands
   const size_t packed_es = split3 ? 3 * 2 : es;    // bytes per packed element (three bf16 planes)
   std::vector<BlockOp> ops(nops);
@@ -107,9 +111,14 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
     j.src = src;
     j.rows = rows;
     j.cols = cols;
-    j.nrows = nrows;
+    // complex blocks are packed realified: L as 2 rows x 2k per row, R as 2k per row
+    j.cplx = cplx ? (role == 0 ? 1 : 2) : 0;
+    j.conj = cplx && (role == 0 ? lconj : rconj);
+    j.nrows = j.cplx == 1 ? 2 * nrows : nrows;
     j.k = k;
-    j.kp = (k + kq - 1) / kq * kq;
+    const int64_t kr = cplx ? 2 * k : k;
+    j.kp = (kr + kq - 1) / kq * kq;
+    nrows = j.nrows;
     j.dst = reinterpret_cast<void *>(packed_bytes);  // offset for now, pointer after allocation
     jobs.push_back(j);
     packed_at[{src, role}] = packed_bytes;
@@ -128,7 +137,7 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
     std::vector<int64_t> ds(dr), ls(lr), rs(rr);
     int64_t st = 1;
     for (int q = 0; q < dr; ++q) { ds[q] = st; st *= dd[q]; }
-    if (st >= (1ll << 31)) return TALSH_LIMIT_EXCEEDED;  // 32-bit offsets inside a D block
+    if (st * (cplx ? 2 : 1) >= (1ll << 31)) return TALSH_LIMIT_EXCEEDED;  // 32-bit offsets inside a D block
     st = 1;
     for (int q = 0; q < lr; ++q) { ls[q] = st; st *= ld[q]; }
     st = 1;
@@ -162,18 +171,20 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
       Kr.s[q] = rs[j];
       k *= ld[i];
     }
-    const int64_t kp = (k + kq - 1) / kq * kq;
-    if (m * kp >= (1ll << 31) || n * kp >= (1ll << 31)) return TALSH_LIMIT_EXCEEDED;
+    const int64_t mr = cplx ? 2 * m : m, kr = cplx ? 2 * k : k;  // realified rows of L, contracted extent
+    const int64_t kp = (kr + kq - 1) / kq * kq;
+    if (mr * kp >= (1ll << 31) || n * kp >= (1ll << 31)) return TALSH_LIMIT_EXCEEDED;
     a_at[o] = add_job(pl, 0, Ml, Kl, m, k);
     b_at[o] = add_job(pr, 1, Nr, Kr, n, k);
     BlockOp &b = ops[o];
-    b.m = m;
+    b.m = mr;
     b.n = n;
     b.kp = kp;
     b.Md = Md;
     b.Nd = Nd;
+    b.cplx = cplx ? 1 : 0;
     dptr[o] = pd;
-    flops += 2.0 * (double)m * (double)n * (double)k;
+    flops += (cplx ? 8.0 : 2.0) * (double)m * (double)n * (double)k;
   }
   int64_t nchunks = 0;
   for (PackJob &j : jobs) {

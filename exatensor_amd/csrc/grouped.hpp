@@ -17,25 +17,37 @@ struct IdxMap {
 
 // Gather of one operand block into a packed k-fast matrix:
 //   dst[r * kp + kk] = src[rows(r) + cols(kk)]  (kk < k), 0 for k <= kk < kp.
-// With the three-way BF16 split (R4 fast math) dst holds three bf16 planes of
-// nrows * kp values each.
+// With the three-way BF16 split (R4/C4 fast math) dst holds three bf16 planes
+// of nrows * kp values each.
+// Complex blocks are packed realified (the real product of the packed
+// matrices is the complex product, interleaved in its rows):
+//   cplx 1 (L operand): packed row 2r   = [ Re x(r,:), -Im x(r,:) ],
+//                       packed row 2r+1 = [ Im x(r,:),  Re x(r,:) ];
+//   cplx 2 (R operand): packed row r    = [ Re x(r,:),  Im x(r,:) ];
+// nrows counts packed rows, k is the source's contracted extent (complex
+// elements) and kp the packed row length (>= 2k); conj negates Im x.
 struct PackJob {
   const void *src;
   void *dst;
   IdxMap rows, cols;
   int64_t nrows, k, kp;
   int64_t chunk0;  // index of this job's first chunk of kPackChunk elements
+  int cplx, conj;
 };
 constexpr int kPackChunk = 4096;
 
 // One block product D(m x n) += A(m x k) B(n x k)^T on packed operands
 // (rows of kp elements, k fastest, zero padded); Md / Nd give the offsets of
 // the composite m / n indices in the D block.
+// A complex block (cplx) runs as the real product of its realified packed
+// operands: m counts packed rows (twice the block's rows) and row 2r + c of
+// the product is component c of D element r.
 struct BlockOp {
   const void *a;
   const void *b;
   int64_t m, n, kp;
   IdxMap Md, Nd;
+  int cplx;
 };
 
 // A tile of one D block and the chain ops[op_begin, op_end) that accumulate
@@ -48,9 +60,10 @@ struct WorkItem {
 
 constexpr int kGroupedTile = 128;
 
-// split3: R4 operands packed as three bf16 planes and multiplied with the
-// three-way split (six MFMA products, FP64 folds); kp must then be a multiple
-// of 32.
+// kind selects the element type (R8/C8 run on FP64, R4/C4 on FP32 matrix
+// cores); split3: R4/C4 operands packed as three bf16 planes and multiplied
+// with the three-way split (six MFMA products, FP64 folds); kp must then be a
+// multiple of 32.
 int launch_pack(int kind, bool split3, const PackJob *jobs, int njobs, int64_t nchunks, int grid, hipStream_t s);
 int launch_grouped(int kind, bool split3, const BlockOp *ops, const WorkItem *items, int nitems, int *counter,
                    double alpha, bool accumulate, int grid, hipStream_t s);

@@ -17,6 +17,9 @@
 //     element is written by one workgroup in chain order: results are
 //     deterministic and need no atomics.
 //
+// Complex blocks (C4/C8) are packed realified (see PackJob) and run through
+// the same real kernels.
+//
 // Replaces the per-block talsh_tensor_contract calls issued by ExaTENSOR's
 // block-level caller (INTERVIRT/tavp_worker.F90:9440-9480).
 #include "bf16x3.hpp"
@@ -92,12 +95,22 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob *__restrict__ jobs, 
     T *dst = static_cast<T *>(J.dst);
     const uint32_t total = (uint32_t)(J.nrows * J.kp), kp = (uint32_t)J.kp, k = (uint32_t)J.k;
     const uint32_t base = (uint32_t)((c - J.chunk0) * kPackChunk);
+    const T isign = J.conj ? T(-1) : T(1);
     for (int e = threadIdx.x; e < kPackChunk; e += 256) {
       const uint32_t idx = base + (uint32_t)e;
       if (idx >= total) break;
       const uint32_t r = idx / kp, kk = idx - r * kp;
       T v = T(0);
-      if (kk < k) v = src[map_off(J.rows, r) + map_off(J.cols, kk)];
+      if (!J.cplx) {
+        if (kk < k) v = src[map_off(J.rows, r) + map_off(J.cols, kk)];
+      } else if (kk < 2 * k) {
+        const uint32_t half = kk >= k ? 1u : 0u, kc = kk - half * k;
+        const uint32_t rc = J.cplx == 1 ? r >> 1 : r, comp = J.cplx == 1 ? (r & 1u) : 0u;
+        const int64_t o = 2 * (map_off(J.rows, rc) + map_off(J.cols, kc));
+        const T re = src[o], im = isign * src[o + 1];
+        // (comp, half): (0,0) re, (0,1) -im (L) or im (R), (1,0) im, (1,1) re
+        v = comp == half ? re : (J.cplx == 1 && half) ? -im : im;
+      }
       if constexpr (SPLIT) {
         unsigned short h0, h1, h2;
         split3(v, h0, h1, h2);
@@ -192,14 +205,15 @@ __global__ __launch_bounds__(256, OCC) void k_grouped_gemm(const BlockOp *__rest
         __syncthreads();
       }
     }
-    // scatter the tile into the D block (index maps of the chain's first op)
+    // scatter the tile into the D block (index maps of the chain's first op;
+    // a complex block's product row 2r + c is component c of element r)
     const BlockOp &o0 = ops[w.op_begin];
     T *D = static_cast<T *>(w.d);
     int32_t coff[FN];
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       const int64_t col = n0 + wn + j * 16 + fr;
-      coff[j] = col < o0.n ? map_off32(o0.Nd, (uint32_t)col) : -1;
+      coff[j] = col < o0.n ? (o0.cplx ? 2 : 1) * map_off32(o0.Nd, (uint32_t)col) : -1;
     }
     const T a = (T)alpha;
 #pragma unroll
@@ -209,7 +223,8 @@ __global__ __launch_bounds__(256, OCC) void k_grouped_gemm(const BlockOp *__rest
         // FP64 16x16x4 keeps rows lane/16 + 4r in a lane, FP32 rows 4(lane/16) + r
         const int64_t row = m0 + wm + i * 16 + (sizeof(T) == 8 ? fk + 4 * r : fk * 4 + r);
         if (row >= o0.m) continue;
-        const int32_t roff = map_off32(o0.Md, (uint32_t)row);
+        const int32_t roff = o0.cplx ? 2 * map_off32(o0.Md, (uint32_t)(row >> 1)) + (int32_t)(row & 1)
+                                     : map_off32(o0.Md, (uint32_t)row);
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
           if (coff[j] < 0) continue;
@@ -350,7 +365,7 @@ __global__ __launch_bounds__(kB3GroupedThreads) void k_grouped_b3(const BlockOp 
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       const int64_t col = n0 + wn + j * 16 + fr;
-      coff[j] = col < o0.n ? map_off32(o0.Nd, (uint32_t)col) : -1;
+      coff[j] = col < o0.n ? (o0.cplx ? 2 : 1) * map_off32(o0.Nd, (uint32_t)col) : -1;
     }
 #pragma unroll
     for (int i = 0; i < FM; ++i)
@@ -358,7 +373,8 @@ __global__ __launch_bounds__(kB3GroupedThreads) void k_grouped_b3(const BlockOp 
       for (int r = 0; r < 4; ++r) {
         const int64_t row = m0 + wm + i * 16 + (lane >> 4) * 4 + r;  // FP32 16x16 layout
         if (row >= o0.m) continue;
-        const int32_t roff = map_off32(o0.Md, (uint32_t)row);
+        const int32_t roff = o0.cplx ? 2 * map_off32(o0.Md, (uint32_t)(row >> 1)) + (int32_t)(row & 1)
+                                     : map_off32(o0.Md, (uint32_t)row);
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
           if (coff[j] < 0) continue;
@@ -375,9 +391,10 @@ __global__ __launch_bounds__(kB3GroupedThreads) void k_grouped_b3(const BlockOp 
 int launch_pack(int kind, bool split3, const PackJob *jobs, int njobs, int64_t nchunks, int grid, hipStream_t s)
 {
   if (njobs <= 0 || nchunks <= 0) return 0;
-  if (kind == R8 && !split3) hipLaunchKernelGGL((k_pack<double, false>), dim3(grid), dim3(256), 0, s, jobs, njobs, nchunks);
-  else if (kind == R4 && !split3) hipLaunchKernelGGL((k_pack<float, false>), dim3(grid), dim3(256), 0, s, jobs, njobs, nchunks);
-  else if (kind == R4) hipLaunchKernelGGL((k_pack<float, true>), dim3(grid), dim3(256), 0, s, jobs, njobs, nchunks);
+  const bool dp = kind == R8 || kind == C8, sp = kind == R4 || kind == C4;
+  if (dp && !split3) hipLaunchKernelGGL((k_pack<double, false>), dim3(grid), dim3(256), 0, s, jobs, njobs, nchunks);
+  else if (sp && !split3) hipLaunchKernelGGL((k_pack<float, false>), dim3(grid), dim3(256), 0, s, jobs, njobs, nchunks);
+  else if (sp) hipLaunchKernelGGL((k_pack<float, true>), dim3(grid), dim3(256), 0, s, jobs, njobs, nchunks);
   else return TALSH_INVALID_ARGS;
   return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
 }
@@ -388,13 +405,13 @@ int launch_grouped(int kind, bool split3, const BlockOp *ops, const WorkItem *it
   if (nitems <= 0) return 0;
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, counter);
   if (split3) {
-    if (kind != R4) return TALSH_INVALID_ARGS;
+    if (kind != R4 && kind != C4) return TALSH_INVALID_ARGS;
     hipLaunchKernelGGL(k_grouped_b3, dim3(grid), dim3(kB3GroupedThreads), 0, s, ops, items, nitems, counter, alpha,
                        accumulate ? 1 : 0);
-  } else if (kind == R8)
+  } else if (kind == R8 || kind == C8)
     hipLaunchKernelGGL((k_grouped_gemm<double, 2>), dim3(grid), dim3(256), 0, s, ops, items, nitems, counter, alpha,
                        accumulate ? 1 : 0);
-  else if (kind == R4)
+  else if (kind == R4 || kind == C4)
     hipLaunchKernelGGL((k_grouped_gemm<float, 2>), dim3(grid), dim3(256), 0, s, ops, items, nitems, counter, alpha,
                        accumulate ? 1 : 0);
   else

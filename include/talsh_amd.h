@@ -57,7 +57,9 @@ int talshAmdGemm(int data_kind, int fast_math, size_t m, size_t n, size_t k, con
 size_t talshAmdGemmWorkspace(int data_kind, int fast_math, size_t m, size_t n, size_t k);
 
 /* Batched block-sparse contraction: nops block contractions
- *   D_i (+)= alpha * L_i * R_i   (one symbolic pattern, R8 or R4, real alpha)
+ *   D_i (+)= alpha * L_i * R_i   (one symbolic pattern, R8, R4, C8 or C4 with
+ *   the pattern's conjugation flags, real alpha; R4/C4 under fast math use
+ *   the three-way BF16 split)
  * whose bodies are all resident on GPU dev_num. Blocks sharing a D tensor
  * accumulate in op order. Create builds the work queue once; Execute runs it
  * synchronously as one persistent grouped launch (use_graph: replayed from a

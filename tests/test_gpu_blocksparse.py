@@ -23,7 +23,9 @@ def _build(kind, segments, fill, max_ops, seed, ext_set=(3, 5, 8, 13, 16, 24)):
     def get(tag, key, modes):
         if (tag, key) not in tens:
             dims = BS.dims(ext, key, modes)
-            arr = rng.uniform(-1, 1, int(np.prod(dims))).astype(dt)
+            n = int(np.prod(dims))
+            arr = rng.uniform(-1, 1, n) + (1j * rng.uniform(-1, 1, n) if kind in (T.C4, T.C8) else 0)
+            arr = arr.astype(dt)
             host[(tag, key)] = (arr, dims)
             tens[(tag, key)] = gpu_tensor(arr, dims)
         return tens[(tag, key)]
@@ -149,5 +151,57 @@ def test_configured_batch_sampled(oracle, kind, fast):
             want = want + oracle.contract_exact(BS.PATTERN, zero, dd, l, ld, r, rd, 0.75, True, pos)
         got = tens[("D", dk)].download()[pos.astype(np.int64)]
         assert rel_err(got, want) <= tol, (dk, len(chains[dk]))
+    for t in tens.values():
+        t.destruct()
+
+
+@pytest.mark.parametrize("kind,fast,pattern", [
+    (T.C8, 0, BS.PATTERN),
+    (T.C8, 0, "D(a,b,c,d)+=L+(a,k,c,l)*R(l,b,k,d)"),
+    (T.C8, 0, "D+(a,b,c,d)+=L(a,k,c,l)*R+(l,b,k,d)"),
+    (T.C4, 0, "D(a,b,c,d)+=L(a,k,c,l)*R+(l,b,k,d)"),
+    (T.C4, 1, BS.PATTERN),
+    (T.C4, 1, "D+(a,b,c,d)+=L(a,k,c,l)*R(l,b,k,d)"),
+])
+def test_complex_batches_match_oracle(oracle, kind, fast, pattern):
+    """Complex block batches: operands packed realified (L as 2m x 2k, R as
+    n x 2k), the real grouped kernels (FP64, FP32, or the three-way BF16
+    split for C4 under fast math), conjugation flags of the pattern applied
+    in the pack; every D block against the oracle: 1e-12 (C8), 1e-5 (C4),
+    1e-6 (C4 split, against complex128 sums)."""
+    eng, ext, ops, host, tens, D, L, R = _build(kind, 4, 0.35, 250, 31 + kind + fast)
+    lib = eng.lib
+    n = len(ops)
+    arrD = (T._TP * n)(*[C.pointer(t.t) for t in D])
+    arrL = (T._TP * n)(*[C.pointer(t.t) for t in L])
+    arrR = (T._TP * n)(*[C.pointer(t.t) for t in R])
+    b = C.c_void_p()
+    if fast:
+        assert lib.talshEnableFastMath(T.DEV_AMD_GPU, 0) == 0
+    try:
+        assert lib.talshAmdBatchCreate(pattern.encode(), n, arrD, arrL, arrR, 0.75, T.YEP, 0, C.byref(b)) == 0
+    finally:
+        if fast:
+            lib.talshDisableFastMath(T.DEV_AMD_GPU, 0)
+    nops, nitems, flops = C.c_int(), C.c_int(), C.c_double()
+    assert lib.talshAmdBatchInfo(b, C.byref(nops), C.byref(nitems), C.byref(flops)) == 0
+    want_flops = 0.0
+    for dk, lk, rk in ops:
+        a_, b_, c_, d_ = BS.dims(ext, dk, "abcd")
+        want_flops += 8.0 * a_ * b_ * c_ * d_ * ext["k"][lk[1]] * ext["l"][lk[3]]
+    assert nops.value == n and flops.value == want_flops
+    assert lib.talshAmdBatchExecute(b, 1) == 0
+    assert lib.talshAmdBatchDestroy(b) == 0
+    cdt = np.complex128 if fast else T.NP_DTYPE[kind]
+    exp = {k: v[0].astype(cdt) for k, v in host.items() if k[0] == "D"}
+    for dk, lk, rk in ops:
+        dd = BS.dims(ext, dk, "abcd")
+        l, ld = host[("L", lk)]
+        r, rd = host[("R", rk)]
+        tmp = exp[("D", dk)]
+        assert oracle.contract(pattern, tmp, dd, l.astype(cdt), ld, r.astype(cdt), rd, 0.75, True) == 0
+    tol = 1e-12 if kind == T.C8 else (1e-6 if fast else 1e-5)
+    for key, val in exp.items():
+        assert rel_err(tens[key].download(), val) <= tol, key
     for t in tens.values():
         t.destruct()
