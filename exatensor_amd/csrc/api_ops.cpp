@@ -1393,34 +1393,75 @@ int talshTensorContractXL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *
     for (int a = 0; a < 3; ++a) stage_bytes = std::max(stage_bytes, slice_bytes(op->tens_slice[a], es));
   void *stage = nullptr;
   if (!e && !XT_HIP_OK(hipHostMalloc(&stage, std::max<size_t>(stage_bytes, 16), hipHostMallocDefault))) e = TRY_LATER;
+  // One piece's slices on the GPU. Pieces run in order; while piece k's
+  // contraction runs (an asynchronous task), the L and R slices of piece k+1
+  // are loaded (GPU slice kernels on other streams, or host gathers and
+  // copies), so two pieces are in flight as in the reference's progress loop
+  // (TALSH/talshc.cpp:5524-5592). D slices load after the previous piece's D
+  // slice is stored: pieces that split a contracted index share D regions
+  // and accumulate in sequence.
+  struct Piece {
+    talsh_tens_t loc[3];
+    std::vector<int> offs[3];
+    bool loaded[3] = {false, false, false};
+  };
+  auto piece_clear = [&](Piece &P) {
+    for (auto &t : P.loc) {
+      if (talshTensorIsEmpty(&t) != YEP) talshTensorDestruct(&t);
+      talshTensorClean(&t);
+    }
+    for (int a = 0; a < 3; ++a) { P.offs[a].clear(); P.loaded[a] = false; }
+  };
+  // constructs and fills argument a of piece P for operation op (blocking)
+  auto piece_load = [&](talsh_tens_op_t *op, Piece &P, int a) -> int {
+    const talsh_tens_slice_t &sl = op->tens_slice[a];
+    const int r = sl.shape.num_dim;
+    P.offs[a].clear();
+    for (int i = 0; i < r; ++i) P.offs[a].push_back((int)sl.bases.offsets[i]);
+    int err2 = talshTensorConstruct(&P.loc[a], kind, r, sl.shape.dims, gpu, nullptr, -1, nullptr, 0.0, 0.0);
+    if (err2) { talshTensorClean(&P.loc[a]); return err2; }
+    if (on_gpu[a]) {
+      err2 = talshTensorSlice(&P.loc[a], args[a], P.offs[a].data(), gpu, DEV_NULL, COPY_MT, NOPE, nullptr);
+    } else {
+      host_box(static_cast<char *>(host[a]), args[a]->shape_p->dims, sl.bases.offsets, sl.shape.dims, r, es,
+               static_cast<char *>(stage), true);
+      if (!XT_HIP_OK(hipMemcpy(image_body(&P.loc[a], gpu, kind), stage, slice_bytes(sl, es), hipMemcpyHostToDevice)))
+        err2 = TALSH_FAILURE;
+    }
+    if (!err2) P.loaded[a] = true;
+    return err2;
+  };
+  Piece cur, nxt;
+  for (auto *P : {&cur, &nxt})
+    for (auto &t : P->loc) talshTensorClean(&t);
   for (size_t k = 0; !e && k < ops.size(); ++k) {
     talsh_tens_op_t *op = ops[k];
-    talsh_tens_t loc[3];
-    for (auto &t : loc) talshTensorClean(&t);
-    std::vector<int> offs[3];
-    for (int a = 0; a < 3 && !e; ++a) {
-      const talsh_tens_slice_t &sl = op->tens_slice[a];
-      const int r = sl.shape.num_dim;
-      for (int i = 0; i < r; ++i) offs[a].push_back((int)sl.bases.offsets[i]);
-      e = talshTensorConstruct(&loc[a], kind, r, sl.shape.dims, gpu, nullptr, -1, nullptr, 0.0, 0.0);
-      if (e) break;
-      // load the slice (D included: pieces accumulate into its current values)
-      if (on_gpu[a]) {
-        e = talshTensorSlice(&loc[a], args[a], offs[a].data(), gpu, DEV_NULL, COPY_MT, NOPE, nullptr);
-      } else {
-        host_box(static_cast<char *>(host[a]), args[a]->shape_p->dims, sl.bases.offsets, sl.shape.dims, r, es,
-                 static_cast<char *>(stage), true);
-        if (!XT_HIP_OK(hipMemcpy(image_body(&loc[a], gpu, kind), stage, slice_bytes(sl, es), hipMemcpyHostToDevice)))
-          e = TALSH_FAILURE;
+    for (int a = 1; a < 3 && !e; ++a)  // L and R unless loaded ahead
+      if (!cur.loaded[a]) e = piece_load(op, cur, a);
+    if (!e) e = piece_load(op, cur, 0);  // D: its current values (pieces accumulate)
+    talsh_task_t tk;
+    talshTaskClean(&tk);
+    if (!e)
+      e = talshTensorContract(cptrn, &cur.loc[0], &cur.loc[1], &cur.loc[2], sr, si, gpu, DEV_NULL, COPY_MTT, YEP,
+                              &tk);
+    if (!e && k + 1 < ops.size()) {  // load ahead: L and R of the next piece
+      for (int a = 1; a < 3; ++a) {
+        const int ea = piece_load(ops[k + 1], nxt, a);
+        if (ea == TRY_LATER || ea == DEVICE_UNABLE) break;  // no room now: loaded in its own turn
+        if (ea) { e = ea; break; }
       }
     }
-    if (!e)
-      e = talshTensorContract(cptrn, &loc[0], &loc[1], &loc[2], sr, si, gpu, DEV_NULL, COPY_MTT, YEP, nullptr);
+    if (talshTaskIsEmpty(&tk) != YEP) {
+      int st = 0;
+      const int ew = talshTaskWait(&tk, &st);
+      if (!e && (ew || st != TALSH_TASK_COMPLETED)) e = ew ? ew : TALSH_TASK_ERROR;
+      talshTaskDestruct(&tk);
+    }
     if (!e) {  // store the D slice
       const talsh_tens_slice_t &sl = op->tens_slice[0];
       if (on_gpu[0]) {
-        e = talshTensorInsert(dtens, &loc[0], offs[0].data(), gpu, DEV_NULL, COPY_MT, NOPE, nullptr);
-      } else if (XT_HIP_OK(hipMemcpy(stage, image_body(&loc[0], gpu, kind), slice_bytes(sl, es),
+        e = talshTensorInsert(dtens, &cur.loc[0], cur.offs[0].data(), gpu, DEV_NULL, COPY_MT, NOPE, nullptr);
+      } else if (XT_HIP_OK(hipMemcpy(stage, image_body(&cur.loc[0], gpu, kind), slice_bytes(sl, es),
                                      hipMemcpyDeviceToHost))) {
         host_box(static_cast<char *>(host[0]), dtens->shape_p->dims, sl.bases.offsets, sl.shape.dims,
                  sl.shape.num_dim, es, static_cast<char *>(stage), false);
@@ -1428,9 +1469,11 @@ int talshTensorContractXL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *
         e = TALSH_FAILURE;
       }
     }
-    for (auto &t : loc)
-      if (talshTensorIsEmpty(&t) != YEP) talshTensorDestruct(&t);
+    piece_clear(cur);
+    std::swap(cur, nxt);
   }
+  piece_clear(cur);
+  piece_clear(nxt);
   if (stage) (void)hipHostFree(stage);
   release();
   return e;
