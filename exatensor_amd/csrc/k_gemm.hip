@@ -707,6 +707,50 @@ __global__ __launch_bounds__(256) void k_splitk_reduce_cx(const float2 *__restri
   }
 }
 
+// Realified complex operands for the native-precision complex product (C8 on
+// FP64, C4 on FP32 matrix cores) of large problems: the same layout as the
+// three-way split's complex form (k' = 2k, k fastest),
+//   A'' (2m x k'):  row 2r   = [ Re a(r,:), -Im a(r,:) ],  row 2r+1 = [ Im a(r,:), Re a(r,:) ]
+//   B'' (n x k'):   row j    = [ Re b(j,:),  Im b(j,:) ]
+// so that the real product A'' B''^T, column-major 2m x n, is the complex m x n
+// product interleaved. Conjugation negates imaginary parts; a complex alpha
+// (are, aim) is applied to A on the way in (the real product then runs with
+// alpha 1). One thread per source element.
+template <typename T, bool IS_A>
+__global__ __launch_bounds__(256) void k_realify(const T *__restrict__ x, int64_t rows, int64_t k, int64_t ld,
+                                                 int kfast, int conj, double are, double aim,
+                                                 T *__restrict__ out)
+{
+  const int64_t k2 = 2 * k;
+  const int64_t total = rows * k;  // source elements
+  const T isign = conj ? T(-1) : T(1);
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (int64_t)gridDim.x * 256) {
+    // i walks the source in its storage order (k fastest or rows fastest),
+    // so that the reads are coalesced
+    int64_t r, kk;
+    if (kfast) { r = i / k; kk = i - r * k; }
+    else { kk = i / rows; r = i - kk * rows; }
+    const int64_t src = kfast ? r * ld + kk : kk * ld + r;
+    T re = x[2 * src], im = isign * x[2 * src + 1];
+    if (IS_A) {
+      if (aim != 0.0 || are != 1.0) {
+        const T nr = (T)are * re - (T)aim * im, ni = (T)are * im + (T)aim * re;
+        re = nr;
+        im = ni;
+      }
+      T *r0 = out + (2 * r) * k2, *r1 = r0 + k2;
+      r0[kk] = re;
+      r0[k + kk] = -im;
+      r1[kk] = im;
+      r1[k + kk] = re;
+    } else {
+      T *r0 = out + r * k2;
+      r0[kk] = re;
+      r0[k + kk] = im;
+    }
+  }
+}
+
 template <typename K>
 int launch_grid(K kern, const GemmParams &p, hipStream_t s)
 {
@@ -865,10 +909,27 @@ int splitk_parts(const GemmArgs &g, int tile, int64_t *kchunk)
   return (int)(((int64_t)g.k + *kchunk - 1) / *kchunk);
 }
 
+// Large complex products in native precision run realified through the real
+// 128x128 kernels (which re-read operands from L2 half as often as the 64x64
+// complex kernel), when the real product has enough 128x128 tiles to fill the
+// chip twice and K is deep enough to amortise the realify pass.
+bool complex_realified(const GemmArgs &g)
+{
+  if ((g.kind != C8 && g.kind != C4) || gemm_uses_split3(g)) return false;
+  const size_t tiles = ((2 * g.m + 127) / 128) * ((g.n + 127) / 128);
+  return tiles >= (size_t)(2 * current_cus()) && g.k >= 512;
+}
+
+size_t realified_bytes(const GemmArgs &g)
+{
+  const size_t es = g.kind == C8 ? 8 : 4, k2 = 2 * g.k;
+  return ((2 * g.m * k2 * es + 255) / 256 * 256) + g.n * k2 * es + 16;
+}
+
 size_t gemm_workspace_bytes(const GemmArgs &g)
 {
   if (!gemm_uses_split3(g)) {
-    if (g.kind == C8 || g.kind == C4) return 0;
+    if (g.kind == C8 || g.kind == C4) return complex_realified(g) ? realified_bytes(g) : 0;
     const int tile = g.tile < 0 ? gemm_default_tile(g.kind, g.r4mode, g.m, g.n, g.k) : g.tile;
     int64_t kc;
     const int nz = splitk_parts(g, tile, &kc);
@@ -960,6 +1021,46 @@ int launch_gemm_ws(const GemmArgs &g, void *ws, size_t ws_bytes, hipStream_t s)
       if (e) return e;
       return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
     }
+  }
+  if (ws && complex_realified(g)) {
+    const bool dp = g.kind == C8;
+    const size_t es = dp ? 8 : 4;
+    const int64_t k2 = 2 * (int64_t)g.k;
+    char *a2 = static_cast<char *>(ws);
+    char *b2 = a2 + (2 * g.m * (size_t)k2 * es + 255) / 256 * 256;
+    const bool real_alpha = g.alpha_im == 0.0;
+    const double are = real_alpha ? 1.0 : g.alpha_re, aim = real_alpha ? 0.0 : g.alpha_im;
+    const int64_t lda = g.a_kfast ? (int64_t)g.k : (int64_t)g.m, ldb = g.b_kfast ? (int64_t)g.k : (int64_t)g.n;
+    const int ga = (int)std::min<int64_t>(((int64_t)g.m * (int64_t)g.k + 255) / 256, 65536);
+    const int gb = (int)std::min<int64_t>(((int64_t)g.n * (int64_t)g.k + 255) / 256, 65536);
+    GemmParams q = p;
+    q.m = 2 * (int64_t)g.m;
+    q.n = (int64_t)g.n;
+    q.k = k2;
+    q.lda = q.ldb = k2;
+    q.ldc = q.m;
+    q.a = a2;
+    q.b = b2;
+    q.conj_a = q.conj_b = 0;
+    q.alpha_re = real_alpha ? g.alpha_re : 1.0;
+    q.alpha_im = 0.0;
+    q.vec_a = q.vec_b = ((k2 * (int64_t)es) % 16) == 0;
+    if (dp) {
+      hipLaunchKernelGGL((k_realify<double, true>), dim3(ga), dim3(256), 0, s, static_cast<const double *>(g.a),
+                         (int64_t)g.m, (int64_t)g.k, lda, g.a_kfast ? 1 : 0, g.conj_a ? 1 : 0, are, aim,
+                         reinterpret_cast<double *>(a2));
+      hipLaunchKernelGGL((k_realify<double, false>), dim3(gb), dim3(256), 0, s, static_cast<const double *>(g.b),
+                         (int64_t)g.n, (int64_t)g.k, ldb, g.b_kfast ? 1 : 0, g.conj_b ? 1 : 0, 1.0, 0.0,
+                         reinterpret_cast<double *>(b2));
+      return launch_family<double, false, 128, 128, 16, 2>(q, true, true, s);
+    }
+    hipLaunchKernelGGL((k_realify<float, true>), dim3(ga), dim3(256), 0, s, static_cast<const float *>(g.a),
+                       (int64_t)g.m, (int64_t)g.k, lda, g.a_kfast ? 1 : 0, g.conj_a ? 1 : 0, are, aim,
+                       reinterpret_cast<float *>(a2));
+    hipLaunchKernelGGL((k_realify<float, false>), dim3(gb), dim3(256), 0, s, static_cast<const float *>(g.b),
+                       (int64_t)g.n, (int64_t)g.k, ldb, g.b_kfast ? 1 : 0, g.conj_b ? 1 : 0, 1.0, 0.0,
+                       reinterpret_cast<float *>(b2));
+    return launch_family<float, false, 128, 128, 16, 2>(q, true, true, s);
   }
   switch (g.kind) {
   case R8:

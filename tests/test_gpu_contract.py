@@ -445,3 +445,24 @@ def test_c4_fast_math_conjugation_alpha_and_plans(oracle):
             finally:
                 L.talshAmdSetPlanVariant(-1, -1)
                 L.talshDisableFastMath(T.DEV_AMD_GPU, 0)
+
+
+@pytest.mark.parametrize("pat", ["D(a,b)+=L(i,a)*R+(b,i)", "D+(a,b)+=L+(a,i)*R(i,b)", "D(b,a)+=L(a,i)*R(b,i)"])
+def test_large_complex_conjugated_contractions(oracle, pat):
+    """C8 contractions large enough for the realified product path, with
+    conjugated operands (applied while realifying) and complex alpha; 1024
+    sampled elements against long-double sums."""
+    sect = lambda s: s[s.index("(") + 1:s.index(")")].split(",")
+    dpart, rest = pat.split("+=")
+    lpart, rpart = rest.split("*")
+    ext = {"a": 2304, "b": 2112, "i": 700}
+    dd, ld, rd = ([ext[x] for x in sect(p)] for p in (dpart, lpart, rpart))
+    l = oracle.normal(T.C8, int(np.prod(ld)), 71)
+    r = oracle.normal(T.C8, int(np.prod(rd)), 72)
+    d = oracle.normal(T.C8, int(np.prod(dd)), 73)
+    alpha = complex(0.6, -0.8)
+    e, got = contract(pat, d, dd, l, ld, r, rd, alpha)
+    assert e == 0
+    pos = np.random.default_rng(74).choice(d.size, 1024, replace=False).astype(np.uint64)
+    ex = oracle.contract_exact(pat, d, dd, l, ld, r, rd, alpha, True, pos)
+    assert rel_err(got[pos.astype(np.int64)], ex) <= 1e-12

@@ -90,11 +90,12 @@ def test_split_k_workspace_is_reported():
     assert L.talshAmdGemmWorkspace(T.C8, 0, 64, 64, 5000) == 0
 
 
-def _gemm_c4(m, n, k, ak, bk, beta_zero, fast, seed=0, alpha=0.75 - 0.4j):
+def _gemm_c4(m, n, k, ak, bk, beta_zero, fast, seed=0, alpha=0.75 - 0.4j, kind=T.C4):
     eng = engine()
     L = eng.lib
     rng = np.random.default_rng(seed)
-    cx = lambda size: (rng.uniform(-1, 1, size) + 1j * rng.uniform(-1, 1, size)).astype(np.complex64)
+    cdt = np.complex64 if kind == T.C4 else np.complex128
+    cx = lambda size: (rng.uniform(-1, 1, size) + 1j * rng.uniform(-1, 1, size)).astype(cdt)
     a, b, c0 = cx(m * k), cx(n * k), cx(m * n)
     A = a.reshape(m, k) if ak else a.reshape(k, m).T
     B = b.reshape(n, k) if bk else b.reshape(k, n).T
@@ -102,10 +103,10 @@ def _gemm_c4(m, n, k, ak, bk, beta_zero, fast, seed=0, alpha=0.75 - 0.4j):
     if not beta_zero:
         want += c0.reshape(n, m).T.astype(np.complex128)
     ta, tb, tc = gpu_tensor(a, [a.size]), gpu_tensor(b, [b.size]), gpu_tensor(c0, [c0.size])
-    wsb = L.talshAmdGemmWorkspace(T.C4, fast, m, n, k)
+    wsb = L.talshAmdGemmWorkspace(kind, fast, m, n, k)
     tw = T.Tensor(L, T.R4, [max(1, (wsb + 3) // 4)], eng.gpu(0), 0.0) if wsb else None
     ptr = lambda t: t.body(eng.gpu(0))[1] if t is not None else None
-    e = L.talshAmdGemm(T.C4, fast, m, n, k, ptr(ta), ak, ptr(tb), bk, ptr(tc), alpha.real, alpha.imag,
+    e = L.talshAmdGemm(kind, fast, m, n, k, ptr(ta), ak, ptr(tb), bk, ptr(tc), alpha.real, alpha.imag,
                        1 if beta_zero else 0, -1, ptr(tw), wsb)
     got = tc.download().reshape(n, m).T if e == 0 else None
     for t in (ta, tb, tc, tw):
@@ -145,3 +146,26 @@ def test_c4_workspace_is_reported():
     np_ = -(-n // 128) * 128
     nz = -(-kp // 8192)
     assert L.talshAmdGemmWorkspace(T.C4, 1, m, n, k) == 3 * (mp + np_) * kp * 2 + nz * 2 * m * n * 4 + 16
+
+
+@pytest.mark.parametrize("kind", [T.C8, T.C4])
+@pytest.mark.parametrize("m,n,k,ak,bk,beta_zero,alpha", [
+    (2048, 2048, 600, 1, 1, True, 0.75),           # realified: 512 real 128x128 tiles
+    (2100, 1950, 1000, 0, 1, False, 0.75 - 0.4j),  # ragged, complex alpha (applied to A), accumulate
+    (2048, 2100, 777, 1, 0, False, -1.5),
+    (2050, 2040, 513, 0, 0, True, 0.3 + 1.1j),
+])
+def test_complex_realified_native(kind, m, n, k, ak, bk, beta_zero, alpha):
+    """Large complex products in native precision run realified (2m x 2k and
+    n x 2k real operands, one real 128x128-tile product written straight into
+    the interleaved complex output): every orientation, complex and real
+    alpha, beta 0/1, against complex128 products; the workspace query reports
+    the realified operands; small products keep the complex kernel (no
+    workspace)."""
+    L = engine().lib
+    es = 16 if kind == T.C8 else 8
+    assert L.talshAmdGemmWorkspace(kind, 0, m, n, k) >= (2 * m + n) * 2 * k * es // 2
+    assert L.talshAmdGemmWorkspace(kind, 0, 200, 200, k) == 0
+    e, got, want, wsb = _gemm_c4(m, n, k, ak, bk, beta_zero, fast=0, seed=m + k, alpha=complex(alpha), kind=kind)
+    assert e == 0 and wsb > 0
+    assert rel_err(got, want) <= (1e-13 if kind == T.C8 else 1e-5)
