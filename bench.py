@@ -222,6 +222,7 @@ def run_contractions(gb, reps=3):
         gemm_tf = flops * gn / (gms / 1e3) / 1e12 if gms > 0 else None
         # outside the timed region: the result of the last (non-accumulating) call against float64 sums
         err = sampled_error_c3(lh, rh, d.download())
+        progress(f"contraction {name}: {tf:.1f} TFLOP/s")
         out[name] = dict(tflops=round(tf, 3), gemm_tflops=round(gemm_tf, 3) if gemm_tf else None,
                          gemm_frac_peak=round(gemm_tf / peak, 4) if gemm_tf else None, peak_tflops=peak,
                          ms_per_op=round((t1 - t0) / reps * 1e3, 3), gemm_ms=round(gms / max(gn, 1), 3),
@@ -332,6 +333,7 @@ def run_blocksparse(gb, reps=3):
         per_block_s = time.perf_counter() - t0
         if fast:
             gb.L.talshDisableFastMath(T.DEV_AMD_GPU, 0)
+        progress(f"block-sparse {name}: grouped {grouped_s * 1e3:.1f} ms, per block {per_block_s * 1e3:.1f} ms")
         out[name] = dict(grouped_blocks_per_s=round(n / grouped_s, 1), grouped_ms=round(grouped_s * 1e3, 3),
                          grouped_kernel_ms=round(kms / max(kn, 1), 3),
                          grouped_tflops=round(out["flops"] / grouped_s / 1e12, 3),
@@ -352,11 +354,13 @@ def run_selector(gb, n=300, seed=303):
     rng = np.random.default_rng(seed)
     specs, times = [], []
     t0 = time.perf_counter()
-    for _ in range(n):
+    for i in range(n):
         spec = S.random_contraction(rng)
         _, _, _, tm = S.time_candidates(gb.L, gb.gpu, spec)
         specs.append(spec)
         times.append(tm)
+        if (i + 1) % 50 == 0:
+            progress(f"plan selector: {i + 1}/{n} contractions timed")
     m = S.Mlp(gb.L, path=S.DEFAULT_MODEL)
     rep = S.regret_report(gb.L, specs, times, m)
     gb.L.talshAmdSelectorInstall(None)
@@ -493,6 +497,11 @@ def cpu_baseline(class_seconds=4.0, contract_seconds=3.0):
     return base, c1
 
 
+def progress(msg):
+    """One line per finished part on stderr (the JSON result stays alone on stdout)."""
+    print(f"bench: {msg} ({time.strftime('%H:%M:%S')})", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -511,13 +520,17 @@ def main():
         sys.exit(2)
     gb = GpuBench()
     tr = run_transposes(gb, args.steps, args.warmup)
+    progress("transposes timed")
     if args.copy_probe:
         copy_probe(gb)
     comps = {} if args.no_contract else run_contractions(gb)
+    progress("contractions timed")
     if not args.no_blocksparse:
         comps["blocksparse"] = run_blocksparse(gb)
+        progress("block-sparse batches timed")
     if not args.no_selector:
         comps["plan_selector"] = run_selector(gb)
+        progress("plan selector timed")
     value = tr["bytes"] / 1e9 / tr["seconds"]
     achieved = tr["achieved_gbs"]
     line = {
