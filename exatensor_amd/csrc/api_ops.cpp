@@ -1758,6 +1758,18 @@ int talshAmdPermutePlan(int data_kind, int rank, const int *dims, const int *o2n
   return 0;
 }
 
+static GemmArgs gemm_args(int data_kind, int fast_math, size_t m, size_t n, size_t k, const void *a, int a_kfast,
+                          const void *b, int b_kfast, void *c, double alpha_re, double alpha_im, int beta_zero, int tile)
+{
+  GemmArgs g{};
+  g.kind = data_kind;
+  g.r4mode = fast_math ? R4Mode::kBF16x3 : R4Mode::kNativeF32;
+  g.m = m; g.n = n; g.k = k; g.a = a; g.b = b; g.c = c;
+  g.a_kfast = a_kfast; g.b_kfast = b_kfast; g.beta_zero = beta_zero;
+  g.alpha_re = alpha_re; g.alpha_im = alpha_im; g.tile = tile;
+  return g;
+}
+
 int talshAmdGemm(int data_kind, int fast_math, size_t m, size_t n, size_t k, const void *a, int a_kfast,
                  const void *b, int b_kfast, void *c, double alpha_re, double alpha_im, int beta_zero, int tile,
                  void *workspace, size_t workspace_bytes)
@@ -1765,12 +1777,8 @@ int talshAmdGemm(int data_kind, int fast_math, size_t m, size_t n, size_t k, con
   Engine &E = engine();
   if (!E.on || E.gpus.empty()) return TALSH_NOT_INITIALIZED;
   Device *d = E.gpus[0];
-  GemmArgs g{};
-  g.kind = data_kind;
-  g.r4mode = fast_math ? R4Mode::kBF16x3 : R4Mode::kNativeF32;
-  g.m = m; g.n = n; g.k = k; g.a = a; g.b = b; g.c = c;
-  g.a_kfast = a_kfast; g.b_kfast = b_kfast; g.beta_zero = beta_zero;
-  g.alpha_re = alpha_re; g.alpha_im = alpha_im; g.tile = tile;
+  const GemmArgs g = gemm_args(data_kind, fast_math, m, n, k, a, a_kfast, b, b_kfast, c, alpha_re, alpha_im,
+                               beta_zero, tile);
   const size_t need = gemm_workspace_bytes(g);
   // the three-way split cannot run without its workspace; a real product
   // without room for split-K partials runs unsplit
@@ -1782,6 +1790,31 @@ int talshAmdGemm(int data_kind, int fast_math, size_t m, size_t n, size_t k, con
   const int e = need <= workspace_bytes ? launch_gemm_ws(g, workspace, workspace_bytes, s)
                                         : launch_gemm_ws(g, nullptr, 0, s);
   prof_end(s, kKGemm, pr);
+  if (e) return e;
+  return XT_HIP_OK(hipStreamSynchronize(s)) ? 0 : TALSH_FAILURE;
+}
+
+// The same product with its workspace in the GPU's fixed scratch array: no
+// allocation, no timing events. Products whose workspace exceeds the array
+// are refused before anything is launched.
+int talshAmdGemmNoAlloc(int data_kind, int fast_math, size_t m, size_t n, size_t k, const void *a, int a_kfast,
+                        const void *b, int b_kfast, void *c, double alpha_re, double alpha_im, int beta_zero, int tile)
+{
+  Engine &E = engine();
+  if (!E.on || E.gpus.empty()) return TALSH_NOT_INITIALIZED;
+  Device *d = E.gpus[0];
+  if (!d->noalloc_scratch) return TALSH_FAILURE;
+  const GemmArgs g = gemm_args(data_kind, fast_math, m, n, k, a, a_kfast, b, b_kfast, c, alpha_re, alpha_im,
+                               beta_zero, tile);
+  const size_t need = gemm_workspace_bytes(g);
+  if (need > kGemmNoAllocScratchBytes) {
+    log_error("talshAmdGemmNoAlloc: the product needs %zu bytes of scratch, more than the fixed %zu", need,
+              kGemmNoAllocScratchBytes);
+    return TALSH_LIMIT_EXCEEDED;
+  }
+  (void)hipSetDevice(d->ordinal);
+  hipStream_t s = d->streams[0];
+  const int e = launch_gemm_ws(g, d->noalloc_scratch, kGemmNoAllocScratchBytes, s);
   if (e) return e;
   return XT_HIP_OK(hipStreamSynchronize(s)) ? 0 : TALSH_FAILURE;
 }

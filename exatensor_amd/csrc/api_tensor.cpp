@@ -422,6 +422,9 @@ int talshInit(size_t *host_buf_size, int *host_arg_max, int ngpus, int gpu_list[
     d->arena.reset(d->arena_mem, want);
     for (int k = 0; k < kStreamsPerDevice; ++k)
       if (!XT_HIP_OK(hipStreamCreateWithFlags(&d->streams[k], hipStreamNonBlocking))) { delete d; return TALSH_FAILURE; }
+    // this GPU's address of the allocation-free product's scratch array (device current)
+    if (gemm_noalloc_scratch(&d->noalloc_scratch))
+      log_error("GPU %d: scratch of talshAmdGemmNoAlloc not found; that entry point will fail", d->ordinal);
     E.gpus.push_back(d);
   }
   if (!E.gpus.empty()) (void)hipSetDevice(E.gpus[0]->ordinal);

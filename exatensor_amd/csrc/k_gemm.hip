@@ -944,6 +944,21 @@ size_t gemm_workspace_bytes(const GemmArgs &g)
 
 int launch_gemm(const GemmArgs &g, hipStream_t s) { return launch_gemm_ws(g, nullptr, 0, s); }
 
+// Scratch of talshAmdGemmNoAlloc. In host code this name is a host-side
+// stand-in, not a device address: only the pointer that
+// gemm_noalloc_scratch returns may be handed to kernels or copies.
+static __device__ __align__(256) unsigned char scratch[kGemmNoAllocScratchBytes];
+
+int gemm_noalloc_scratch(void **ptr)
+{
+  *ptr = nullptr;
+  if (!XT_HIP_OK(hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch)))) {
+    *ptr = nullptr;
+    return TALSH_FAILURE;
+  }
+  return 0;
+}
+
 bool gemm_uses_split3(const GemmArgs &g) { return (g.kind == R4 || g.kind == C4) && g.r4mode == R4Mode::kBF16x3; }
 
 int gemm_tile_choices(int kind, R4Mode r4mode);

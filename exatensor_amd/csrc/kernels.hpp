@@ -69,6 +69,13 @@ size_t gemm_workspace_bytes(const GemmArgs &g);
 // ws holds ws_bytes; a workspace smaller than gemm_workspace_bytes(g) is an
 // error for the three-way split and runs a real product unsplit.
 int launch_gemm_ws(const GemmArgs &g, void *ws, size_t ws_bytes, hipStream_t s);
+// Fixed per-GPU scratch of the allocation-free product entry point: a
+// file-scope device array in k_gemm.hip (64 MiB, held by every GPU the
+// library runs on until the process ends). gemm_noalloc_scratch returns its
+// address on the current device (hipGetSymbolAddress; call once per device,
+// with that device current, and keep the pointer per device).
+constexpr size_t kGemmNoAllocScratchBytes = size_t(64) << 20;
+int gemm_noalloc_scratch(void **ptr);
 // Number of tile-shape choices of launch_gemm for a kind, and the output tile
 // (bm x bn) and occupancy target of a choice.
 int gemm_tile_choices(int kind, R4Mode r4mode);

@@ -32,6 +32,7 @@ struct Device {
   std::atomic<unsigned> next_stream{0};
   DeviceStats stats;
   R4Mode r4mode = R4Mode::kNativeF32;
+  void *noalloc_scratch = nullptr;   // this GPU's copy of the allocation-free product's scratch array
 };
 
 struct Engine {

@@ -139,6 +139,8 @@ def load_library(path=LIB_PATH):
                                    C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int,
                                    C.c_void_p, C.c_size_t]),
         "talshAmdGemmWorkspace": (C.c_size_t, [C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t]),
+        "talshAmdGemmNoAlloc": (C.c_int, [C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p, C.c_int,
+                                          C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int]),
         "talshAmdBatchCreate": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_TP), C.POINTER(_TP), C.POINTER(_TP),
                                           C.c_double, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
         "talshAmdBatchExecute": (C.c_int, [C.c_void_p, C.c_int]),

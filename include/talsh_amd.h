@@ -55,6 +55,17 @@ int talshAmdGemm(int data_kind, int fast_math, size_t m, size_t n, size_t k, con
                  const void *b, int b_kfast, void *c, double alpha_re, double alpha_im, int beta_zero, int tile,
                  void *workspace, size_t workspace_bytes);
 size_t talshAmdGemmWorkspace(int data_kind, int fast_math, size_t m, size_t n, size_t k);
+/* talshAmdGemm for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference. The workspace is a fixed 64 MiB device
+ * array per GPU (held by each GPU the library runs on until the process
+ * ends); a product whose talshAmdGemmWorkspace() need exceeds 64 MiB returns
+ * TALSH_LIMIT_EXCEEDED before anything runs. Calls no allocator and records
+ * no timing events. Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (other host threads) race. Synchronous. */
+int talshAmdGemmNoAlloc(int data_kind, int fast_math, size_t m, size_t n, size_t k, const void *a, int a_kfast,
+                        const void *b, int b_kfast, void *c, double alpha_re, double alpha_im, int beta_zero,
+                        int tile);
 
 /* Batched block-sparse contraction: nops block contractions
  *   D_i (+)= alpha * L_i * R_i   (one symbolic pattern, R8, R4, C8 or C4 with
