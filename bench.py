@@ -257,7 +257,7 @@ def _finish(gb, tk):
     gb.L.talshTaskDestruct(C.byref(tk))
 
 
-def run_blocksparse(gb, reps=3):
+def run_blocksparse(gb, reps=3, per_block_leg=True):
     """C4: ~4096 uneven block contractions of D(a,b,c,d)+=L(a,k,c,l)*R(l,b,k,d)
     (7 segments per mode, extents from {8..64}, 20% block fill), as one
     persistent grouped launch replayed from a hipGraph, against one
@@ -323,22 +323,24 @@ def run_blocksparse(gb, reps=3):
             for tk in inflight:
                 _finish(gb, tk)
 
-        # an untimed pass first: it fills the plan memos (the autotuner plans
-        # block shapes outside the learned selector's domain on first sight)
-        per_block(n)
-        gb.sync()
-        t0 = time.perf_counter()
-        per_block(n)
-        gb.sync()
-        per_block_s = time.perf_counter() - t0
-        if fast:
-            gb.L.talshDisableFastMath(T.DEV_AMD_GPU, 0)
-        progress(f"block-sparse {name}: grouped {grouped_s * 1e3:.1f} ms, per block {per_block_s * 1e3:.1f} ms")
         out[name] = dict(grouped_blocks_per_s=round(n / grouped_s, 1), grouped_ms=round(grouped_s * 1e3, 3),
                          grouped_kernel_ms=round(kms / max(kn, 1), 3),
-                         grouped_tflops=round(out["flops"] / grouped_s / 1e12, 3),
-                         per_block_blocks_per_s=round(n / per_block_s, 1), per_block_ms=round(per_block_s * 1e3, 3),
-                         speedup=round(per_block_s / grouped_s, 2))
+                         grouped_tflops=round(out["flops"] / grouped_s / 1e12, 3))
+        if per_block_leg:
+            # an untimed pass first: it fills the plan memos (the autotuner plans
+            # block shapes outside the learned selector's domain on first sight)
+            per_block(n)
+            gb.sync()
+            t0 = time.perf_counter()
+            per_block(n)
+            gb.sync()
+            per_block_s = time.perf_counter() - t0
+            out[name].update(per_block_blocks_per_s=round(n / per_block_s, 1),
+                             per_block_ms=round(per_block_s * 1e3, 3), speedup=round(per_block_s / grouped_s, 2))
+        if fast:
+            gb.L.talshDisableFastMath(T.DEV_AMD_GPU, 0)
+        progress(f"block-sparse {name}: grouped {grouped_s * 1e3:.1f} ms"
+                 + (f", per block {per_block_s * 1e3:.1f} ms" if per_block_leg else ""))
         for t in tens.values():
             t.destruct()
     return out
@@ -511,6 +513,8 @@ def main():
     ap.add_argument("--no-contract", action="store_true", help="skip the contraction components")
     ap.add_argument("--no-blocksparse", action="store_true", help="skip the block-sparse component")
     ap.add_argument("--no-selector", action="store_true", help="skip the plan-selector regret component")
+    ap.add_argument("--no-per-block", action="store_true",
+                    help="block-sparse component without its one-task-per-block baseline leg (for counter runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, run identity copies of known size (counter calibration)")
     args = ap.parse_args()
@@ -526,7 +530,7 @@ def main():
     comps = {} if args.no_contract else run_contractions(gb)
     progress("contractions timed")
     if not args.no_blocksparse:
-        comps["blocksparse"] = run_blocksparse(gb)
+        comps["blocksparse"] = run_blocksparse(gb, per_block_leg=not args.no_per_block)
         progress("block-sparse batches timed")
     if not args.no_selector:
         comps["plan_selector"] = run_selector(gb)

@@ -29,7 +29,8 @@ struct Batch {
   Device *dev = nullptr;
   int kind = NO_TYPE;
   int nops = 0, nitems = 0, njobs = 0;
-  int64_t nchunks = 0;
+  int64_t nchunks = 0, ntiles = 0;
+  bool cplx = false;         // complex blocks: element-wise realifying pack
   double alpha = 1.0;
   bool acc = true;
   int grid = 0, pack_grid = 0;
@@ -47,7 +48,8 @@ struct Batch {
 
 int launch_batch(const Batch *B, hipStream_t s)
 {
-  int e = launch_pack(B->kind, B->split3, B->d_jobs, B->njobs, B->nchunks, B->pack_grid, s);
+  int e = B->cplx ? launch_pack(B->kind, B->split3, B->d_jobs, B->njobs, B->nchunks, B->pack_grid, s)
+                  : launch_pack_tiled(B->kind, B->split3, B->d_jobs, B->njobs, B->ntiles, B->pack_grid, s);
   if (e) return e;
   return launch_grouped(B->kind, B->split3, B->d_ops, B->d_items, B->nitems, B->d_counter, B->alpha, B->acc,
                         B->grid, s);
@@ -85,6 +87,19 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
   for (int j = 0; j < rr; ++j)
     if (dig[lr + j] > 0) ndims_r.push_back(j);
   if (mdims_l.size() > 4 || ndims_r.size() > 4 || kdims_l.size() > 4) return TALSH_NOT_IMPLEMENTED;
+  // order of the contracted indices inside the packed k index (any order is
+  // the same product, both operands share it): an operand whose first
+  // (stride-1) dimension is contracted reads its source contiguously only if
+  // that index is the fastest one, so R's, else L's, goes first
+  {
+    int first = -1;
+    if (dig[lr] < 0) first = -dig[lr] - 1;  // L position of R's first dimension
+    else if (dig[0] < 0) first = 0;
+    if (first >= 0) {
+      auto it = std::find(kdims_l.begin(), kdims_l.end(), first);
+      if (it != kdims_l.end()) std::rotate(kdims_l.begin(), it, it + 1);
+    }
+  }
   for (int i : mdims_l)
     for (int j : ndims_r)
       if (dig[i] == dig[lr + j]) return TALSH_INVALID_ARGS;  // hyper-index
@@ -118,6 +133,8 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
     j.k = k;
     const int64_t kr = cplx ? 2 * k : k;
     j.kp = (kr + kq - 1) / kq * kq;
+    // tiled pack: lanes along the side whose first dimension has the smaller source stride
+    j.lane_rows = cols.n == 0 || (rows.n > 0 && rows.s[0] <= cols.s[0]) ? 1 : 0;
     nrows = j.nrows;
     j.dst = reinterpret_cast<void *>(packed_bytes);  // offset for now, pointer after allocation
     jobs.push_back(j);
@@ -186,11 +203,15 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
     dptr[o] = pd;
     flops += (cplx ? 8.0 : 2.0) * (double)m * (double)n * (double)k;
   }
-  int64_t nchunks = 0;
+  int64_t nchunks = 0, ntiles = 0;
   for (PackJob &j : jobs) {
     j.chunk0 = nchunks;
     nchunks += (j.nrows * j.kp + kPackChunk - 1) / kPackChunk;
+    j.tile0 = ntiles;
+    j.tiles_k = (int)((j.kp + kPackTile - 1) / kPackTile);
+    ntiles += (int64_t)j.tiles_k * ((j.nrows + kPackTile - 1) / kPackTile);
   }
+  if (ntiles >= (1ll << 31)) return TALSH_LIMIT_EXCEEDED;
   // chains: ops grouped by D body, op order kept inside a chain
   std::vector<int> order(nops);
   std::iota(order.begin(), order.end(), 0);
@@ -224,13 +245,15 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
   B->nitems = (int)sitems.size();
   B->njobs = (int)jobs.size();
   B->nchunks = nchunks;
+  B->ntiles = ntiles;
+  B->cplx = cplx;
   B->alpha = alpha_re;
   B->acc = accumulative != NOPE;
   B->flops = flops;
   B->split3 = split3;
   // resident workgroups per CU: two (FP32/FP64 tiles), one (bf16 planes fill the LDS)
   B->grid = std::min(B->nitems, d->num_cus * (split3 ? 1 : 2));
-  B->pack_grid = (int)std::min<int64_t>(nchunks, (int64_t)d->num_cus * 8);
+  B->pack_grid = (int)std::min<int64_t>(cplx ? nchunks : ntiles, (int64_t)d->num_cus * 8);
   int e = image_alloc(d->flat_id, std::max<size_t>(packed_bytes, 256), &B->packed);
   if (e) { delete B; return e; }
   char *pk = static_cast<char *>(B->packed.gmem_p);

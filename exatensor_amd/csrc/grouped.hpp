@@ -33,8 +33,14 @@ struct PackJob {
   int64_t nrows, k, kp;
   int64_t chunk0;  // index of this job's first chunk of kPackChunk elements
   int cplx, conj;
+  // tiled pack (real blocks): first tile of this job, tiles along the packed
+  // row (kp), and whether lanes read along rows (the rows' first dimension is
+  // the source's contiguous one) or along k
+  int64_t tile0;
+  int tiles_k, lane_rows;
 };
 constexpr int kPackChunk = 4096;
+constexpr int kPackTile = 64;  // tiled pack: 64 packed rows x 64 packed k-values
 
 // One block product D(m x n) += A(m x k) B(n x k)^T on packed operands
 // (rows of kp elements, k fastest, zero padded); Md / Nd give the offsets of
@@ -65,6 +71,9 @@ constexpr int kGroupedTile = 128;
 // with the three-way split (six MFMA products, FP64 folds); kp must then be a
 // multiple of 32.
 int launch_pack(int kind, bool split3, const PackJob *jobs, int njobs, int64_t nchunks, int grid, hipStream_t s);
+// Real (R8/R4) blocks only: the same packing in LDS-staged tiles.
+int launch_pack_tiled(int kind, bool split3, const PackJob *jobs, int njobs, int64_t ntiles, int grid,
+                      hipStream_t s);
 int launch_grouped(int kind, bool split3, const BlockOp *ops, const WorkItem *items, int nitems, int *counter,
                    double alpha, bool accumulate, int grid, hipStream_t s);
 

@@ -125,6 +125,72 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob *__restrict__ jobs, 
   }
 }
 
+// Packs real blocks in tiles of 64 packed rows x 64 packed k-values staged
+// through LDS. The tile's 64 row offsets and 64 column offsets in the source
+// block are decoded once per tile, so an element costs one load, one LDS
+// write, one LDS read and one store; the source is read with lanes along its
+// contiguous side (rows or k, chosen per job on the host) and every packed
+// row segment is written whole, 64 consecutive values per wave.
+template <typename T, bool SPLIT>
+__global__ __launch_bounds__(256) void k_pack_tiled(const PackJob *__restrict__ jobs, int njobs, int64_t ntiles)
+{
+  constexpr int TS = kPackTile;
+  __shared__ T tile[TS][TS + 1];
+  __shared__ int64_t roff[TS], coff[TS];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].tile0 <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const PackJob &J = jobs[lo];
+    const uint32_t local = (uint32_t)(t - J.tile0), tk_n = (uint32_t)J.tiles_k;
+    const uint32_t tr = local / tk_n, tk = local - tr * tk_n;
+    const int64_t r0 = (int64_t)tr * TS, k0 = (int64_t)tk * TS;
+    if (tid < TS) {
+      const int64_t kk = k0 + tid;
+      coff[tid] = kk < J.k ? map_off(J.cols, (uint32_t)kk) : -1;  // k <= kk < kp: zero padding
+    } else if (tid < 2 * TS) {
+      const int64_t r = r0 + (tid - TS);
+      roff[tid - TS] = r < J.nrows ? map_off(J.rows, (uint32_t)r) : -1;
+    }
+    __syncthreads();
+    const T *src = static_cast<const T *>(J.src);
+    const bool lr = J.lane_rows != 0;
+#pragma unroll 4
+    for (int q = 0; q < TS / 4; ++q) {
+      const int b = wave + 4 * q;
+      const int i = lr ? lane : b, j = lr ? b : lane;
+      const int64_t ro = roff[i], co = coff[j];
+      tile[i][j] = (ro >= 0 && co >= 0) ? src[ro + co] : T(0);
+    }
+    __syncthreads();
+    const int64_t kp = J.kp, total = J.nrows * kp;
+#pragma unroll 4
+    for (int q = 0; q < TS / 4; ++q) {
+      const int i = wave + 4 * q;
+      const int64_t r = r0 + i, kk = k0 + lane;
+      if (r < J.nrows && kk < kp) {
+        const T v = tile[i][lane];
+        const int64_t idx = r * kp + kk;
+        if constexpr (SPLIT) {
+          unsigned short h0, h1, h2;
+          split3(v, h0, h1, h2);
+          unsigned short *d16 = static_cast<unsigned short *>(J.dst);
+          d16[idx] = h0;
+          d16[total + idx] = h1;
+          d16[2 * total + idx] = h2;
+        } else {
+          static_cast<T *>(J.dst)[idx] = v;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 template <typename T, int OCC>
 __global__ __launch_bounds__(256, OCC) void k_grouped_gemm(const BlockOp *__restrict__ ops,
                                                            const WorkItem *__restrict__ items, int nitems,
@@ -396,6 +462,21 @@ int launch_pack(int kind, bool split3, const PackJob *jobs, int njobs, int64_t n
   else if (sp && !split3) hipLaunchKernelGGL((k_pack<float, false>), dim3(grid), dim3(256), 0, s, jobs, njobs, nchunks);
   else if (sp) hipLaunchKernelGGL((k_pack<float, true>), dim3(grid), dim3(256), 0, s, jobs, njobs, nchunks);
   else return TALSH_INVALID_ARGS;
+  return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
+}
+
+int launch_pack_tiled(int kind, bool split3, const PackJob *jobs, int njobs, int64_t ntiles, int grid,
+                      hipStream_t s)
+{
+  if (njobs <= 0 || ntiles <= 0) return 0;
+  if (kind == R8 && !split3)
+    hipLaunchKernelGGL((k_pack_tiled<double, false>), dim3(grid), dim3(256), 0, s, jobs, njobs, ntiles);
+  else if (kind == R4 && !split3)
+    hipLaunchKernelGGL((k_pack_tiled<float, false>), dim3(grid), dim3(256), 0, s, jobs, njobs, ntiles);
+  else if (kind == R4)
+    hipLaunchKernelGGL((k_pack_tiled<float, true>), dim3(grid), dim3(256), 0, s, jobs, njobs, ntiles);
+  else
+    return TALSH_INVALID_ARGS;
   return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
 }
 
