@@ -140,7 +140,9 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
     jobs.push_back(j);
     packed_at[{src, role}] = packed_bytes;
     const size_t at = packed_bytes;
-    packed_bytes += ((size_t)(nrows * j.kp) * packed_es + 255) / 256 * 256;  // 256-byte aligned sub-buffers
+    // split planes are tile-blocked in bands of 16 rows
+    const int64_t prow = split3 ? (nrows + 15) / 16 * 16 : nrows;
+    packed_bytes += ((size_t)(prow * j.kp) * packed_es + 255) / 256 * 256;  // 256-byte aligned sub-buffers
     return at;
   };
   std::vector<size_t> a_at(nops), b_at(nops);

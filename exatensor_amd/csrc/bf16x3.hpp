@@ -38,4 +38,46 @@ __device__ __forceinline__ f4 mfma_bf(const us8 &a, const us8 &b, f4 c)
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
 }
 
+// Tile-blocked split planes: blocks of RB rows x KB k-values laid out as a
+// matrix kernel's LDS stage holds them, so that one wave-wide 16-byte DMA
+// moves a whole block (RB * KB * 2 bytes, 1 KB for 16 x 32) from memory
+// into LDS. Lane l of a block holds row l / (KB / 8) at 16-byte slot
+// l % (KB / 8).
+template <int RB, int KB>
+struct B3Layout {
+  static constexpr int kChunks = KB / 8;                  // 16-byte chunks per block row
+  // chunk c of row r sits at c ^ ((r >> kShift) & (kChunks - 1)); for the
+  // 16-row blocks, row bits 1-2 are what leaves the fragment reads free of
+  // bank conflicts (SQ_LDS_BANK_CONFLICT 0, against 4 cycles per ds_read_b128
+  // with bits 0-1, 2-3 or 3-4, and 6% more matrix throughput)
+  static constexpr int kShift = 1;
+  __host__ __device__ static int pos(int r, int c) { return c ^ ((r >> kShift) & (kChunks - 1)); }
+};
+
+// Wave-wide 16-byte DMA from global memory straight into LDS: lane i's 16
+// bytes land at lds_wave_base + 16 i.
+__device__ __forceinline__ void lds_dma16(const void *g, void *lds_wave_base)
+{
+  __builtin_amdgcn_global_load_lds(g, lds_wave_base, 16, 0, 0);
+}
+
+// s_waitcnt with only the vector-memory count (expcnt and lgkmcnt at their
+// no-wait maxima), gfx9 encoding.
+template <int N>
+__device__ __forceinline__ void wait_vmcnt()
+{
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+// Barrier without the release/acquire fences of __syncthreads (whose
+// workgroup-scope release waits for every outstanding load, including the
+// DMA of later stages); the callers order LDS with explicit vmcnt waits. The
+// empty asm statements keep the compiler from moving memory accesses across.
+__device__ __forceinline__ void stage_barrier()
+{
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
 }  // namespace xt

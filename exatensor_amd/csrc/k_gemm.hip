@@ -304,17 +304,6 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
 constexpr int kB3Chunk = 8192;
 constexpr int kB3M = 256, kB3N = 128;   // output tile of one workgroup (8 waves)
 
-template <int RB, int KB>
-struct B3Layout {
-  static constexpr int kChunks = KB / 8;                  // 16-byte chunks per block row
-  // chunk c of row r sits at c ^ ((r >> kShift) & (kChunks - 1)); for the
-  // 16-row blocks, row bits 1-2 are what leaves the fragment reads free of
-  // bank conflicts (SQ_LDS_BANK_CONFLICT 0, against 4 cycles per ds_read_b128
-  // with bits 0-1, 2-3 or 3-4, and 6% more matrix throughput)
-  static constexpr int kShift = 1;
-  __host__ __device__ static int pos(int r, int c) { return c ^ ((r >> kShift) & (kChunks - 1)); }
-};
-
 // Split pass: FP32 operand -> three bf16 planes in the tile-blocked layout,
 // zero-padded to rows_pad x k_pad, so the matrix kernel needs no bounds
 // checks. Both forms write whole 1 KB blocks per wave (lane l fills the
@@ -485,30 +474,6 @@ __device__ __forceinline__ void tile_of_b3(const B3Params &p, int &tm, int &tn)
   q.tiles_n = p.tiles_n;
   q.xcd_remap = 1;
   tile_of<4>(q, tm, tn);
-}
-
-__device__ __forceinline__ void lds_dma16(const void *g, void *lds_wave_base)
-{
-  __builtin_amdgcn_global_load_lds(g, lds_wave_base, 16, 0, 0);
-}
-
-// s_waitcnt with only the vector-memory count (expcnt and lgkmcnt at their
-// no-wait maxima), gfx9 encoding.
-template <int N>
-__device__ __forceinline__ void wait_vmcnt()
-{
-  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
-}
-
-// Barrier without the release/acquire fences of __syncthreads (whose
-// workgroup-scope release waits for every outstanding load, including the
-// DMA of later stages); the callers order LDS with explicit vmcnt waits. The
-// empty asm statements keep the compiler from moving memory accesses across.
-__device__ __forceinline__ void stage_barrier()
-{
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_barrier();
-  asm volatile("" ::: "memory");
 }
 
 // Issue the DMA of one k-stage: NL 1 KB blocks (A planes x row bands, then B),

@@ -22,6 +22,8 @@
 //
 // Replaces the per-block talsh_tensor_contract calls issued by ExaTENSOR's
 // block-level caller (INTERVIRT/tavp_worker.F90:9440-9480).
+#include <type_traits>
+
 #include "bf16x3.hpp"
 #include "grouped.hpp"
 
@@ -78,6 +80,19 @@ __device__ __forceinline__ int32_t map_off32(const IdxMap &mp, uint32_t v)
 
 __global__ void k_zero_counter(int *c) { *c = 0; }
 
+// Three-way split operands are packed tile-blocked (B3Layout<16, 32>): row
+// band b (16 rows) and k-block q (32 values) form the 1 KB block at
+// b * 16 * kp + q * 512 of each plane; a plane holds round_up(rows, 16) * kp
+// values (the padding rows are never stored: they feed only product rows or
+// columns that are not written back).
+typedef B3Layout<16, 32> PackLay;
+__device__ __forceinline__ size_t b3_plane(int64_t nrows, int64_t kp) { return (size_t)((nrows + 15) / 16 * 16 * kp); }
+__device__ __forceinline__ size_t b3_packed_offset(uint32_t r, uint32_t kk, int64_t kp)
+{
+  const uint32_t rr = r & 15u, q = kk >> 5, c = (kk >> 3) & 3u;
+  return (size_t)(r >> 4) * 16 * kp + (size_t)q * 512 + rr * 32 + PackLay::pos((int)rr, (int)c) * 8 + (kk & 7u);
+}
+
 // Packs chunks of kPackChunk destination elements; a chunk belongs to the job
 // with the largest chunk0 not above it (binary search over the sorted jobs).
 template <typename T, bool SPLIT>
@@ -115,9 +130,10 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob *__restrict__ jobs, 
         unsigned short h0, h1, h2;
         split3(v, h0, h1, h2);
         unsigned short *d16 = static_cast<unsigned short *>(J.dst);
-        d16[idx] = h0;
-        d16[(size_t)total + idx] = h1;
-        d16[2 * (size_t)total + idx] = h2;
+        const size_t o = b3_packed_offset(r, kk, kp), plane = b3_plane(J.nrows, kp);
+        d16[o] = h0;
+        d16[plane + o] = h1;
+        d16[2 * plane + o] = h2;
       } else {
         dst[idx] = v;
       }
@@ -167,24 +183,37 @@ __global__ __launch_bounds__(256) void k_pack_tiled(const PackJob *__restrict__ 
       tile[i][j] = (ro >= 0 && co >= 0) ? src[ro + co] : T(0);
     }
     __syncthreads();
-    const int64_t kp = J.kp, total = J.nrows * kp;
-#pragma unroll 4
-    for (int q = 0; q < TS / 4; ++q) {
-      const int i = wave + 4 * q;
-      const int64_t r = r0 + i, kk = k0 + lane;
-      if (r < J.nrows && kk < kp) {
-        const T v = tile[i][lane];
-        const int64_t idx = r * kp + kk;
-        if constexpr (SPLIT) {
-          unsigned short h0, h1, h2;
-          split3(v, h0, h1, h2);
-          unsigned short *d16 = static_cast<unsigned short *>(J.dst);
-          d16[idx] = h0;
-          d16[total + idx] = h1;
-          d16[2 * total + idx] = h2;
-        } else {
-          static_cast<T *>(J.dst)[idx] = v;
+    const int64_t kp = J.kp;
+    if constexpr (SPLIT) {
+      // the tile is 4 row bands x 2 k-blocks of the tile-blocked planes; each
+      // wave splits two whole 1 KB blocks, lane l filling slot l of each
+      const int64_t nbands = (J.nrows + 15) / 16, nkb = kp / 32;
+      const size_t plane = b3_plane(J.nrows, kp);
+      unsigned short *d16 = static_cast<unsigned short *>(J.dst);
+      const int rr = lane >> 2, c = PackLay::pos(rr, lane & 3);
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int blk = wave * 2 + u, band = blk >> 1, kbl = blk & 1;
+        const int64_t gband = r0 / 16 + band, gkb = k0 / 32 + kbl;
+        if (gband >= nbands || gkb >= nkb) continue;
+        us8 h0, h1, h2;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          unsigned short e0, e1, e2;
+          split3((float)tile[band * 16 + rr][kbl * 32 + c * 8 + e], e0, e1, e2);
+          h0[e] = e0; h1[e] = e1; h2[e] = e2;
         }
+        const size_t off = (size_t)gband * 16 * kp + (size_t)gkb * 512 + lane * 8;
+        *reinterpret_cast<us8 *>(d16 + off) = h0;
+        *reinterpret_cast<us8 *>(d16 + plane + off) = h1;
+        *reinterpret_cast<us8 *>(d16 + 2 * plane + off) = h2;
+      }
+    } else {
+#pragma unroll 4
+      for (int q = 0; q < TS / 4; ++q) {
+        const int i = wave + 4 * q;
+        const int64_t r = r0 + i, kk = k0 + lane;
+        if (r < J.nrows && kk < kp) static_cast<T *>(J.dst)[r * kp + kk] = tile[i][lane];
       }
     }
     __syncthreads();
@@ -302,33 +331,38 @@ __global__ __launch_bounds__(256, OCC) void k_grouped_gemm(const BlockOp *__rest
   }
 }
 
-// Three-way BF16 split form of k_grouped_gemm for R4 under fast math: packed
-// operands are three bf16 planes (plane stride rows * kp); each k-tile of 32
-// runs six 16x16x32 bf16 MFMA products per output block, and the FP32
-// accumulator is folded into an FP64 sum every 8 k-tiles (the chain's k-tiles
-// count together). One workgroup per CU (double-buffered planes fill the LDS)
-// of 8 waves, 2 x 4 over the tile (64 x 32 each), so that two waves share
-// each SIMD and one's MFMAs cover the other's LDS reads.
+// Three-way BF16 split form of k_grouped_gemm (R4 and C4 under fast math).
+// The packed operands are tile-blocked split planes (see b3_packed_offset),
+// so each k-stage of 32 is 48 whole 1 KB blocks (3 planes x 8 row bands of A
+// and of B) that the 8 waves move with wave-wide DMA straight into LDS, six
+// each, into a ring of three stage buffers (144 KB: two stages in flight
+// while one is multiplied); the stage boundary is an explicit vmcnt wait and
+// a bare barrier. A work item's chain is one pipeline: the DMA of the next
+// k-stage may belong to the next block product of the chain. Each stage runs
+// six 16x16x32 bf16 MFMA products per output block; the FP32 accumulator is
+// folded into an FP64 sum every 8 stages. Waves are 2 x 4 over the 128 x 128
+// tile (64 x 32 each, two per SIMD). Row bands beyond an operand's last are
+// read from its last band (those product rows and columns are not stored).
 constexpr int kB3GroupedThreads = 512;
 __global__ __launch_bounds__(kB3GroupedThreads) void k_grouped_b3(const BlockOp *__restrict__ ops,
                                                                   const WorkItem *__restrict__ items, int nitems,
                                                                   int *__restrict__ counter, double alpha,
                                                                   int accumulate)
 {
-  constexpr int NT = kB3GroupedThreads;
-  constexpr int BM = kGroupedTile, BN = kGroupedTile, BK = 32, KP = BK + 8;
+  constexpr int BM = kGroupedTile, BN = kGroupedTile, KB = 32, RB = 16;
   constexpr int WM = BM / 2, WN = BN / 4;      // wave sub-tile
   constexpr int FM = WM / 16, FN = WN / 16;    // 16x16 MFMA blocks per wave
-  constexpr int CA = 3 * BM * (BK / 8) / NT;   // 16-byte chunks of A (3 planes) per thread per k-tile
-  constexpr int CB = 3 * BN * (BK / 8) / NT;
-  static_assert(CA * NT == 3 * BM * (BK / 8) && CB * NT == 3 * BN * (BK / 8), "staging split");
+  constexpr int NBA = BM / RB, NBB = BN / RB;  // row bands per stage and plane
+  constexpr int NBLK = 3 * (NBA + NBB);        // 1 KB blocks per stage
+  constexpr int PER = NBLK / 8;                // DMA blocks per wave per stage
+  static_assert(PER * 8 == NBLK, "stage blocks spread evenly over the waves");
+  constexpr int kStage = NBLK * 512;           // bf16 values per LDS stage
   constexpr int kFold = 8;
-  __shared__ __align__(16) unsigned short As[2][3][BM][KP];
-  __shared__ __align__(16) unsigned short Bs[2][3][BN][KP];
+  __shared__ __align__(16) unsigned short lds[3 * kStage];
   __shared__ int next_item;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
-  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  const int fr = lane & 15, fpos = PackLay::pos(fr, lane >> 4) * 8;
   while (true) {
     if (tid == 0) next_item = atomicAdd(counter, 1);
     __syncthreads();
@@ -337,6 +371,37 @@ __global__ __launch_bounds__(kB3GroupedThreads) void k_grouped_b3(const BlockOp 
     if (it >= nitems) break;
     const WorkItem w = items[it];
     const int64_t m0 = (int64_t)w.mt * BM, n0 = (int64_t)w.nt * BN;
+    // k-stages of the whole chain
+    int64_t nstages = 0;
+    for (int oi = w.op_begin; oi < w.op_end; ++oi) nstages += ops[oi].kp / KB;
+    // DMA cursor: block product and k-stage of the next stage to issue
+    int iop = w.op_begin;
+    int64_t ikb = 0;
+    auto issue = [&](unsigned short *__restrict__ buf) {
+      const BlockOp &op = ops[iop];
+      const int64_t kp = op.kp, bands_a = (op.m + RB - 1) / RB, bands_b = (op.n + RB - 1) / RB;
+      const int64_t pa = bands_a * RB * kp, pb = bands_b * RB * kp;  // plane strides
+      const unsigned short *A = static_cast<const unsigned short *>(op.a);
+      const unsigned short *B = static_cast<const unsigned short *>(op.b);
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int idx = wave + 8 * u;
+        const unsigned short *g;
+        if (idx < 3 * NBA) {
+          const int64_t band = min(m0 / RB + idx % NBA, bands_a - 1);
+          g = A + (idx / NBA) * pa + band * RB * kp;
+        } else {
+          const int jb = idx - 3 * NBA;
+          const int64_t band = min(n0 / RB + jb % NBB, bands_b - 1);
+          g = B + (jb / NBB) * pb + band * RB * kp;
+        }
+        lds_dma16(g + ikb * (RB * KB) + lane * 8, buf + idx * 512);
+      }
+      if (++ikb == kp / KB) {
+        ++iop;
+        ikb = 0;
+      }
+    };
     f4 acc[FM][FN];
     double sum[FM][FN][4];
 #pragma unroll
@@ -347,84 +412,59 @@ __global__ __launch_bounds__(kB3GroupedThreads) void k_grouped_b3(const BlockOp 
 #pragma unroll
         for (int r = 0; r < 4; ++r) sum[i][j][r] = 0.0;
       }
-    int64_t tiles_done = 0;
-    for (int oi = w.op_begin; oi < w.op_end; ++oi) {
-      const BlockOp &op = ops[oi];
-      const int64_t kp = op.kp;
-      const unsigned short *A = static_cast<const unsigned short *>(op.a);
-      const unsigned short *B = static_cast<const unsigned short *>(op.b);
-      const int64_t pa = op.m * kp, pb = op.n * kp;  // plane strides
-      us8 ra[CA], rb[CB];
-      auto load = [&](int64_t k0) {
+    // stage st lives in buffer st % 3; the loop is unrolled over the three
+    // buffers so that every LDS address is a constant offset into one array,
+    // which lets the compiler see that the fragment reads of one buffer do
+    // not depend on the DMA still in flight into the others (otherwise it
+    // waits for all outstanding DMA before the first read of every stage)
+    if (nstages > 0) issue(lds);
+    if (nstages > 1) issue(lds + kStage);
+    auto stage = [&](auto J, int64_t st) {
+      constexpr int j = decltype(J)::value;
+      const unsigned short *cur = lds + j * kStage;
+      if (st + 1 < nstages) wait_vmcnt<PER>();  // stage st landed (st + 1 may still fly)
+      else wait_vmcnt<0>();
+      stage_barrier();  // everyone's stage st landed; stage st - 1's buffer is free
+      if (st + 2 < nstages) issue(lds + ((j + 2) % 3) * kStage);
+      const unsigned short *rb = cur + 3 * BM * KB;
+      us8 fb[3][FN];
 #pragma unroll
-        for (int i = 0; i < CA; ++i) {
-          const int c = tid + i * NT, pl = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
-          const int64_t row = m0 + r;
-          ra[i] = row < op.m ? *reinterpret_cast<const us8 *>(A + pl * pa + row * kp + k0 + kc * 8) : us8{};
+      for (int pl = 0; pl < 3; ++pl)
+#pragma unroll
+        for (int jj = 0; jj < FN; ++jj)
+          fb[pl][jj] = *reinterpret_cast<const us8 *>(rb + (pl * BN + wn + jj * 16 + fr) * KB + fpos);
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        us8 fa[3];
+#pragma unroll
+        for (int pl = 0; pl < 3; ++pl) fa[pl] = *reinterpret_cast<const us8 *>(cur + (pl * BM + wm + i * 16 + fr) * KB + fpos);
+        // products outer, blocks inner; smallest terms first
+#pragma unroll
+        for (int t = 0; t < 6; ++t) {
+          const int qa = t == 0 ? 0 : t == 1 ? 1 : t == 2 ? 2 : t == 3 ? 0 : t == 4 ? 1 : 0;
+          const int qb = t == 0 ? 2 : t == 1 ? 1 : t == 2 ? 0 : t == 3 ? 1 : t == 4 ? 0 : 0;
+#pragma unroll
+          for (int jj = 0; jj < FN; ++jj) acc[i][jj] = mfma_bf(fa[qa], fb[qb][jj], acc[i][jj]);
         }
-#pragma unroll
-        for (int i = 0; i < CB; ++i) {
-          const int c = tid + i * NT, pl = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
-          const int64_t row = n0 + r;
-          rb[i] = row < op.n ? *reinterpret_cast<const us8 *>(B + pl * pb + row * kp + k0 + kc * 8) : us8{};
-        }
-      };
-      auto store = [&](int buf) {
-#pragma unroll
-        for (int i = 0; i < CA; ++i) {
-          const int c = tid + i * NT, pl = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
-          *reinterpret_cast<us8 *>(&As[buf][pl][r][kc * 8]) = ra[i];
-        }
-#pragma unroll
-        for (int i = 0; i < CB; ++i) {
-          const int c = tid + i * NT, pl = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
-          *reinterpret_cast<us8 *>(&Bs[buf][pl][r][kc * 8]) = rb[i];
-        }
-      };
-      const int64_t nk = kp / BK;
-      load(0);
-      store(0);
-      __syncthreads();
-      for (int64_t kt = 0; kt < nk; ++kt) {
-        const int cur = kt & 1;
-        if (kt + 1 < nk) load((kt + 1) * BK);
-        us8 fb[3][FN];
-#pragma unroll
-        for (int pl = 0; pl < 3; ++pl)
-#pragma unroll
-          for (int j = 0; j < FN; ++j) fb[pl][j] = *reinterpret_cast<const us8 *>(&Bs[cur][pl][wn + j * 16 + fr][fk]);
-#pragma unroll
-        for (int i = 0; i < FM; ++i) {
-          us8 fa[3];
-#pragma unroll
-          for (int pl = 0; pl < 3; ++pl) fa[pl] = *reinterpret_cast<const us8 *>(&As[cur][pl][wm + i * 16 + fr][fk]);
-#pragma unroll
-          for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[0], fb[2][j], acc[i][j]);
-#pragma unroll
-          for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[1], fb[1][j], acc[i][j]);
-#pragma unroll
-          for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[2], fb[0][j], acc[i][j]);
-#pragma unroll
-          for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[0], fb[1][j], acc[i][j]);
-#pragma unroll
-          for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[1], fb[0][j], acc[i][j]);
-#pragma unroll
-          for (int j = 0; j < FN; ++j) acc[i][j] = mfma_bf(fa[0], fb[0][j], acc[i][j]);
-        }
-        if (++tiles_done % kFold == 0) {
-#pragma unroll
-          for (int i = 0; i < FM; ++i)
-#pragma unroll
-            for (int j = 0; j < FN; ++j) {
-#pragma unroll
-              for (int r = 0; r < 4; ++r) sum[i][j][r] += (double)acc[i][j][r];
-              acc[i][j] = f4{0, 0, 0, 0};
-            }
-        }
-        if (kt + 1 < nk) store(cur ^ 1);
-        __syncthreads();
       }
+      if ((st + 1) % kFold == 0) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int jj = 0; jj < FN; ++jj) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sum[i][jj][r] += (double)acc[i][jj][r];
+            acc[i][jj] = f4{0, 0, 0, 0};
+          }
+      }
+    };
+    for (int64_t st = 0; st < nstages; st += 3) {
+      stage(std::integral_constant<int, 0>{}, st);
+      if (st + 1 < nstages) stage(std::integral_constant<int, 1>{}, st + 1);
+      if (st + 2 < nstages) stage(std::integral_constant<int, 2>{}, st + 2);
     }
+    // the LDS stages are reused by the next item only after every wave has
+    // passed the item's first barrier, which follows these reads
     const BlockOp &o0 = ops[w.op_begin];
     float *D = static_cast<float *>(w.d);
     int32_t coff[FN];
