@@ -80,6 +80,20 @@ __device__ __forceinline__ int32_t map_off32(const IdxMap &mp, uint32_t v)
 
 __global__ void k_zero_counter(int *c) { *c = 0; }
 
+// D blocks are reached through generic pointers stored in the work items;
+// accessed through a global-address-space pointer they compile to global
+// (not flat) loads and stores, which count on the vector-memory counter only.
+template <typename T>
+using GlobalPtr = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ GlobalPtr<T> as_global(void *p)
+{
+  return (GlobalPtr<T>)(static_cast<T *>(p));
+}
+
+// s_waitcnt vmcnt(0) with the LDS and export counters left free (gfx9 encoding).
+__device__ __forceinline__ void wait_all_vmem() { __builtin_amdgcn_s_waitcnt(0 | (7 << 4) | (15 << 8)); }
+
 // Three-way split operands are packed tile-blocked (B3Layout<16, 32>): row
 // band b (16 rows) and k-block q (32 values) form the 1 KB block at
 // b * 16 * kp + q * 512 of each plane; a plane holds round_up(rows, 16) * kp
@@ -255,16 +269,15 @@ __global__ __launch_bounds__(256, OCC) void k_grouped_gemm(const BlockOp *__rest
       const BlockOp &op = ops[oi];
       const int64_t kp = op.kp;
       const bool aok = m0 + s_r < op.m, bok = n0 + s_r < op.n;
-      const T *ap = static_cast<const T *>(op.a) + (aok ? (m0 + s_r) * kp + s_k : 0);
-      const T *bp = static_cast<const T *>(op.b) + (bok ? (n0 + s_r) * kp + s_k : 0);
+      const GlobalPtr<const T> ap = as_global<const T>(const_cast<void *>(op.a)) + (aok ? (m0 + s_r) * kp + s_k : 0);
+      const GlobalPtr<const T> bp = as_global<const T>(const_cast<void *>(op.b)) + (bok ? (n0 + s_r) * kp + s_k : 0);
       T ra[LA], rb[LA];
       auto load = [&](int64_t k0) {
 #pragma unroll
         for (int i = 0; i < LA; i += VW) {
-          *reinterpret_cast<float4 *>(&ra[i]) =
-              aok ? *reinterpret_cast<const float4 *>(ap + k0 + i) : float4{0.f, 0.f, 0.f, 0.f};
-          *reinterpret_cast<float4 *>(&rb[i]) =
-              bok ? *reinterpret_cast<const float4 *>(bp + k0 + i) : float4{0.f, 0.f, 0.f, 0.f};
+          typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+          *reinterpret_cast<u4 *>(&ra[i]) = aok ? *reinterpret_cast<GlobalPtr<const u4>>(ap + k0 + i) : u4{0, 0, 0, 0};
+          *reinterpret_cast<u4 *>(&rb[i]) = bok ? *reinterpret_cast<GlobalPtr<const u4>>(bp + k0 + i) : u4{0, 0, 0, 0};
         }
       };
       auto store = [&](int buf) {
@@ -303,7 +316,7 @@ __global__ __launch_bounds__(256, OCC) void k_grouped_gemm(const BlockOp *__rest
     // scatter the tile into the D block (index maps of the chain's first op;
     // a complex block's product row 2r + c is component c of element r)
     const BlockOp &o0 = ops[w.op_begin];
-    T *D = static_cast<T *>(w.d);
+    const GlobalPtr<T> D = as_global<T>(w.d);
     int32_t coff[FN];
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
@@ -311,23 +324,42 @@ __global__ __launch_bounds__(256, OCC) void k_grouped_gemm(const BlockOp *__rest
       coff[j] = col < o0.n ? (o0.cplx ? 2 : 1) * map_off32(o0.Nd, (uint32_t)col) : -1;
     }
     const T a = (T)alpha;
+    // per 16-row block of the wave: the offsets (-1: outside the D block),
+    // then every D load of the block at once (clamped addresses, no
+    // lane-masked branches), one wait, then the stores. A load-add-store per
+    // element waits for each load in turn behind the stores already issued.
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+    for (int i = 0; i < FM; ++i) {
+      int32_t off[4][FN];
+      T dv[4][FN];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         // FP64 16x16x4 keeps rows lane/16 + 4r in a lane, FP32 rows 4(lane/16) + r
         const int64_t row = m0 + wm + i * 16 + (sizeof(T) == 8 ? fk + 4 * r : fk * 4 + r);
-        if (row >= o0.m) continue;
-        const int32_t roff = o0.cplx ? 2 * map_off32(o0.Md, (uint32_t)(row >> 1)) + (int32_t)(row & 1)
+        const int32_t roff = row >= o0.m ? -1
+                           : o0.cplx ? 2 * map_off32(o0.Md, (uint32_t)(row >> 1)) + (int32_t)(row & 1)
                                      : map_off32(o0.Md, (uint32_t)row);
 #pragma unroll
-        for (int j = 0; j < FN; ++j) {
-          if (coff[j] < 0) continue;
-          T v = a * acc[i][j][r];
-          if (accumulate) v += D[roff + coff[j]];
-          D[roff + coff[j]] = v;
-        }
+        for (int j = 0; j < FN; ++j) off[r][j] = (roff < 0 || coff[j] < 0) ? -1 : roff + coff[j];
       }
+      if (accumulate) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) dv[r][j] = D[max(off[r][j], 0)];
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) dv[r][j] = T(0);
+      }
+      wait_all_vmem();  // on every lane, before the lane-masked stores
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          if (off[r][j] >= 0) D[off[r][j]] = a * acc[i][j][r] + dv[r][j];
+    }
   }
 }
 
@@ -361,8 +393,10 @@ __global__ __launch_bounds__(kB3GroupedThreads) void k_grouped_b3(const BlockOp 
   __shared__ __align__(16) unsigned short lds[3 * kStage];
   __shared__ int next_item;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
-  const int fr = lane & 15, fpos = PackLay::pos(fr, lane >> 4) * 8;
+  // the masks bound the fragment offsets inside one stage for the compiler,
+  // which otherwise waits for the DMA into the other stages
+  const int wm = (wave & 1) * WM, wn = ((wave >> 1) & 3) * WN;
+  const int fr = lane & 15, fpos = PackLay::pos(fr, (lane >> 4) & 3) * 8;
   while (true) {
     if (tid == 0) next_item = atomicAdd(counter, 1);
     __syncthreads();
@@ -466,29 +500,47 @@ __global__ __launch_bounds__(kB3GroupedThreads) void k_grouped_b3(const BlockOp 
     // the LDS stages are reused by the next item only after every wave has
     // passed the item's first barrier, which follows these reads
     const BlockOp &o0 = ops[w.op_begin];
-    float *D = static_cast<float *>(w.d);
+    const GlobalPtr<float> D = as_global<float>(w.d);
     int32_t coff[FN];
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       const int64_t col = n0 + wn + j * 16 + fr;
       coff[j] = col < o0.n ? (o0.cplx ? 2 : 1) * map_off32(o0.Nd, (uint32_t)col) : -1;
     }
+    // per 16-row block of the wave: offsets, then the block's D loads, one
+    // wait, then the stores (see k_grouped_gemm)
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+    for (int i = 0; i < FM; ++i) {
+      int32_t off[4][FN];
+      float dv[4][FN];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t row = m0 + wm + i * 16 + (lane >> 4) * 4 + r;  // FP32 16x16 layout
-        if (row >= o0.m) continue;
-        const int32_t roff = o0.cplx ? 2 * map_off32(o0.Md, (uint32_t)(row >> 1)) + (int32_t)(row & 1)
+        const int64_t row = m0 + wm + i * 16 + ((lane >> 4) & 3) * 4 + r;  // FP32 16x16 layout
+        const int32_t roff = row >= o0.m ? -1
+                           : o0.cplx ? 2 * map_off32(o0.Md, (uint32_t)(row >> 1)) + (int32_t)(row & 1)
                                      : map_off32(o0.Md, (uint32_t)row);
 #pragma unroll
-        for (int j = 0; j < FN; ++j) {
-          if (coff[j] < 0) continue;
-          double v = alpha * (sum[i][j][r] + (double)acc[i][j][r]);
-          if (accumulate) v += (double)D[roff + coff[j]];
-          D[roff + coff[j]] = (float)v;
-        }
+        for (int j = 0; j < FN; ++j) off[r][j] = (roff < 0 || coff[j] < 0) ? -1 : roff + coff[j];
       }
+      if (accumulate) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) dv[r][j] = D[max(off[r][j], 0)];
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) dv[r][j] = 0.f;
+      }
+      wait_all_vmem();  // on every lane, before the lane-masked stores
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          if (off[r][j] >= 0)
+            D[off[r][j]] = (float)(alpha * (sum[i][j][r] + (double)acc[i][j][r]) + (double)dv[r][j]);
+    }
   }
 }
 
