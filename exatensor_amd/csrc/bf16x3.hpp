@@ -69,6 +69,9 @@ __device__ __forceinline__ void wait_vmcnt()
   __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
 }
 
+// s_waitcnt vmcnt(0) with the LDS and export counters left free.
+__device__ __forceinline__ void wait_all_vmem() { wait_vmcnt<0>(); }
+
 // Barrier without the release/acquire fences of __syncthreads (whose
 // workgroup-scope release waits for every outstanding load, including the
 // DMA of later stages); the callers order LDS with explicit vmcnt waits. The

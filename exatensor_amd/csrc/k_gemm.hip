@@ -254,9 +254,15 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
   T *C = static_cast<T *>(p.c);
   T *Pz = split ? static_cast<T *>(p.part) + (int64_t)blockIdx.y * p.m * p.n : nullptr;
   const T ar = (T)p.alpha_re, ai = (T)p.alpha_im;
-  const int fk = lane >> 4;
+  const int fk = (lane >> 4) & 3;
+  // Per 16-row block of the wave: the offsets (-1: outside C), then every C
+  // load of the block at once (clamped addresses, no lane-masked branches),
+  // one wait, then the stores; a load-add-store per element would wait for
+  // each load in turn behind the stores already issued.
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
+    int64_t off[FN][4];
+    T cv[FN][4][NP];
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       const int64_t col = n0 + wn + j * 16 + fr;
@@ -265,24 +271,40 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
         // accumulator layout: FP32 16x16x4 keeps rows 4*(lane/16)+r in a
         // lane, FP64 16x16x4 keeps rows lane/16 + 4*r
         const int64_t row = m0 + wm + i * 16 + (sizeof(T) == 8 ? fk + 4 * r : fk * 4 + r);
-        if (row < p.m && col < p.n) {
-          const int64_t off = row + col * p.ldc;
-          if (CPLX) {
-            const T xr = acc[0][i][j][r], xi = acc[1 % NP][i][j][r];
-            T vr = ar * xr - ai * xi, vi = ar * xi + ai * xr;
-            if (!p.beta_zero) { vr += C[2 * off]; vi += C[2 * off + 1]; }
-            C[2 * off] = vr;
-            C[2 * off + 1] = vi;
-          } else if (split) {
-            Pz[row + col * p.m] = acc[0][i][j][r];
-          } else {
-            T v = ar * acc[0][i][j][r];
-            if (!p.beta_zero) v += C[off];
-            C[off] = v;
-          }
-        }
+        off[j][r] = (row < p.m && col < p.n) ? (split ? row + col * p.m : row + col * p.ldc) : -1;
       }
     }
+    if (!split && !p.beta_zero) {
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int q = 0; q < NP; ++q) cv[j][r][q] = C[NP * (off[j][r] < 0 ? 0 : off[j][r]) + q];
+      wait_all_vmem();  // on every lane, before the lane-masked stores
+    } else {
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int q = 0; q < NP; ++q) cv[j][r][q] = T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (off[j][r] < 0) continue;
+        if (CPLX) {
+          const T xr = acc[0][i][j][r], xi = acc[1 % NP][i][j][r];
+          C[2 * off[j][r]] = ar * xr - ai * xi + cv[j][r][0];
+          C[2 * off[j][r] + 1] = ar * xi + ai * xr + cv[j][r][1 % NP];
+        } else if (split) {
+          Pz[off[j][r]] = acc[0][i][j][r];
+        } else {
+          C[off[j][r]] = ar * acc[0][i][j][r] + cv[j][r][0];
+        }
+      }
   }
 }
 

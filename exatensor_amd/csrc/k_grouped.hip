@@ -91,8 +91,6 @@ __device__ __forceinline__ GlobalPtr<T> as_global(void *p)
   return (GlobalPtr<T>)(static_cast<T *>(p));
 }
 
-// s_waitcnt vmcnt(0) with the LDS and export counters left free (gfx9 encoding).
-__device__ __forceinline__ void wait_all_vmem() { __builtin_amdgcn_s_waitcnt(0 | (7 << 4) | (15 << 8)); }
 
 // Three-way split operands are packed tile-blocked (B3Layout<16, 32>): row
 // band b (16 rows) and k-block q (32 values) form the 1 KB block at
