@@ -655,19 +655,24 @@ int plan(int es, int rank, const int *dims, const int *o2n, Planned &pl, int cus
         if (tout[jj] == i) P.rag_out[slot] = jj;
     }
   }
-  // grid: walk input dims in order (tile index decoded fastest along dim 0)
+  // grid: tile index decoded fastest along the output's minor dimension, then
+  // in output order, so that the tiles in flight at once (consecutive indices
+  // across the persistent grid) write neighbouring regions of the output.
+  // Measured against input order on the 174-transpose workload (+1.9%
+  // geometric mean) and on 40 random shapes of ranks 2-6 (+5.9%).
   P.ng = f.rank;
-  for (int i = 0; i < f.rank; ++i) {
+  for (int q = 0; q < f.rank; ++q) {
+    const int i = n2o[q];
     const uint32_t t = (uint32_t)std::max(tp.t[i], 1);
     const uint32_t n = (uint32_t)((f.ext[i] + t - 1) / t);
-    P.gdiv[i] = to_dev(n);
-    P.g_is[i] = istr[i] * t;
-    P.g_os[i] = ostr[i] * t;
-    P.g_t[i] = t;
-    P.g_ext[i] = (uint32_t)f.ext[i];
-    P.g_rag[i] = -1;
+    P.gdiv[q] = to_dev(n);
+    P.g_is[q] = istr[i] * t;
+    P.g_os[q] = ostr[i] * t;
+    P.g_t[q] = t;
+    P.g_ext[q] = (uint32_t)f.ext[i];
+    P.g_rag[q] = -1;
     for (int k = 0; k < P.nrag; ++k)
-      if (tin[P.rag_in[k]] == i) P.g_rag[i] = k;
+      if (tin[P.rag_in[k]] == i) P.g_rag[q] = k;
   }
   P.vol_tile = (uint32_t)tp.vol;
   P.ntiles = (uint32_t)ntiles;
