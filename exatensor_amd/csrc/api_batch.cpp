@@ -3,7 +3,7 @@
 // A batch is built once from arrays of D/L/R block tensors that share one
 // contraction pattern (every tensor resident on the GPU) and can then be
 // executed many times, optionally as a captured hipGraph (pack gather of the
-// distinct L and R blocks, counter reset, persistent grouped GEMM). Blocks
+// distinct L and R blocks, work-queue reset, persistent grouped GEMM). Blocks
 // that share a D tensor form a chain that one work item accumulates in op
 // order.
 #include <algorithm>
@@ -36,12 +36,12 @@ struct Batch {
   int grid = 0, pack_grid = 0;
   bool split3 = false;       // R4 under fast math: three-way BF16 split
   double flops = 0;
-  talsh_dev_rsc_t mem{};     // ops + items + pack jobs + counter in the arena
+  talsh_dev_rsc_t mem{};     // ops + items + pack jobs + work queues in the arena
   talsh_dev_rsc_t packed{};  // packed L and R blocks
   BlockOp *d_ops = nullptr;
   WorkItem *d_items = nullptr;
   PackJob *d_jobs = nullptr;
-  int *d_counter = nullptr;
+  WorkQueues *d_queues = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -51,8 +51,7 @@ int launch_batch(const Batch *B, hipStream_t s)
   int e = B->cplx ? launch_pack(B->kind, B->split3, B->d_jobs, B->njobs, B->nchunks, B->pack_grid, s)
                   : launch_pack_tiled(B->kind, B->split3, B->d_jobs, B->njobs, B->ntiles, B->pack_grid, s);
   if (e) return e;
-  return launch_grouped(B->kind, B->split3, B->d_ops, B->d_items, B->nitems, B->d_counter, B->alpha, B->acc,
-                        B->grid, s);
+  return launch_grouped(B->kind, B->split3, B->d_ops, B->d_items, B->d_queues, B->alpha, B->acc, B->grid, s);
 }
 
 void *gpu_body(talsh_tens_t *t, int flat, int kind)
@@ -218,8 +217,12 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
   std::vector<int> order(nops);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return dptr[a] < dptr[b]; });
-  std::vector<WorkItem> items;
-  std::vector<double> cost;
+  // D blocks (chains) as groups of 128 x 128 work items; a group's items
+  // share operand panels (an item row shares its L panel, a column its R
+  // panel), and groups whose chains start with the same L block share it
+  struct Group { int i, j, tm, tn; double cost; size_t key; void *d; };
+  std::vector<Group> groups;
+  double total_cost = 0;
   for (int i = 0; i < nops;) {
     int j = i;
     while (j < nops && dptr[order[j]] == dptr[order[i]]) ++j;
@@ -227,19 +230,30 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
     double kc = 0;
     for (int q = i; q < j; ++q) kc += (double)ops[order[q]].kp;
     const int tm = (int)((b.m + kGroupedTile - 1) / kGroupedTile), tn = (int)((b.n + kGroupedTile - 1) / kGroupedTile);
-    for (int a = 0; a < tm; ++a)
-      for (int c = 0; c < tn; ++c) {
-        items.push_back(WorkItem{i, j, a, c, dptr[order[i]]});
-        cost.push_back(kc);
-      }
+    groups.push_back(Group{i, j, tm, tn, kc * tm * tn, a_at[order[i]], dptr[order[i]]});
+    total_cost += kc * tm * tn;
     i = j;
   }
-  // longest work first
-  std::vector<int> io(items.size());
-  std::iota(io.begin(), io.end(), 0);
-  std::stable_sort(io.begin(), io.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-  std::vector<WorkItem> sitems(items.size());
-  for (size_t i = 0; i < items.size(); ++i) sitems[i] = items[io[i]];
+  std::stable_sort(groups.begin(), groups.end(), [](const Group &x, const Group &y) { return x.key < y.key; });
+  // one queue per XCD: contiguous runs of groups of about equal cost
+  WorkQueues wq{};
+  std::vector<WorkItem> sitems;
+  {
+    double cum = 0;
+    int qx = 0;
+    wq.begin[0] = 0;
+    for (const Group &g : groups) {
+      while (qx < kQueues - 1 && cum >= total_cost * (qx + 1) / kQueues) {
+        wq.end[qx] = (int)sitems.size();
+        wq.begin[++qx] = (int)sitems.size();
+      }
+      for (int a = 0; a < g.tm; ++a)
+        for (int c = 0; c < g.tn; ++c) sitems.push_back(WorkItem{g.i, g.j, a, c, g.d});
+      cum += g.cost;
+    }
+    wq.end[qx] = (int)sitems.size();
+    for (int x = qx + 1; x < kQueues; ++x) wq.begin[x] = wq.end[x] = (int)sitems.size();
+  }
   Batch *B = new Batch();
   B->dev = d;
   B->kind = kind;
@@ -267,7 +281,7 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
   }
   for (PackJob &j : jobs) j.dst = pk + reinterpret_cast<size_t>(j.dst);
   const size_t ob = sizeof(BlockOp) * nops, ib = sizeof(WorkItem) * sitems.size(), jb = sizeof(PackJob) * jobs.size();
-  const size_t bytes = ob + ib + jb + 256;
+  const size_t bytes = ob + ib + jb + (sizeof(WorkQueues) + 255) / 256 * 256;
   e = image_alloc(d->flat_id, bytes, &B->mem);
   if (e) {
     image_release(B->packed, nullptr);
@@ -278,11 +292,12 @@ int talshAmdBatchCreate(const char *cptrn, int nops, talsh_tens_t *const *dtens,
   B->d_ops = reinterpret_cast<BlockOp *>(base);
   B->d_items = reinterpret_cast<WorkItem *>(base + ob);
   B->d_jobs = reinterpret_cast<PackJob *>(base + ob + ib);
-  B->d_counter = reinterpret_cast<int *>(base + bytes - 256);
+  B->d_queues = reinterpret_cast<WorkQueues *>(base + ob + ib + jb);
   (void)hipSetDevice(d->ordinal);
   bool ok = XT_HIP_OK(hipMemcpy(B->d_ops, sorted.data(), ob, hipMemcpyHostToDevice));
   ok = ok && XT_HIP_OK(hipMemcpy(B->d_items, sitems.data(), ib, hipMemcpyHostToDevice));
   ok = ok && XT_HIP_OK(hipMemcpy(B->d_jobs, jobs.data(), jb, hipMemcpyHostToDevice));
+  ok = ok && XT_HIP_OK(hipMemcpy(B->d_queues, &wq, sizeof(wq), hipMemcpyHostToDevice));
   if (!ok) {
     image_release(B->mem, nullptr);
     image_release(B->packed, nullptr);

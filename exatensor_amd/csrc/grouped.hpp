@@ -66,6 +66,17 @@ struct WorkItem {
 
 constexpr int kGroupedTile = 128;
 
+// Work queues of a grouped launch, one per XCD (each XCD has its own L2): a
+// workgroup takes items from the queue of its XCD (workgroup ids are dealt
+// to the XCDs round robin) and, once that is empty, from the others. Each
+// queue is a contiguous range of the work-item list holding whole D blocks,
+// so the tiles that share operand panels meet in one L2.
+constexpr int kQueues = 8;
+struct WorkQueues {
+  int counter[kQueues];  // items taken from each queue (reset before every launch)
+  int begin[kQueues], end[kQueues];
+};
+
 // kind selects the element type (R8/C8 run on FP64, R4/C4 on FP32 matrix
 // cores); split3: R4/C4 operands packed as three bf16 planes and multiplied
 // with the three-way split (six MFMA products, FP64 folds); kp must then be a
@@ -74,7 +85,7 @@ int launch_pack(int kind, bool split3, const PackJob *jobs, int njobs, int64_t n
 // Real (R8/R4) blocks only: the same packing in LDS-staged tiles.
 int launch_pack_tiled(int kind, bool split3, const PackJob *jobs, int njobs, int64_t ntiles, int grid,
                       hipStream_t s);
-int launch_grouped(int kind, bool split3, const BlockOp *ops, const WorkItem *items, int nitems, int *counter,
+int launch_grouped(int kind, bool split3, const BlockOp *ops, const WorkItem *items, WorkQueues *queues,
                    double alpha, bool accumulate, int grid, hipStream_t s);
 
 }  // namespace xt

@@ -10,7 +10,7 @@
 //     GEMM reads 16-byte vectors along k with no bounds checks on k.
 //  2. k_grouped_gemm is persistent: each workgroup pulls work items (a
 //     128x128 tile of one D block plus the chain of (L, R) pairs that
-//     accumulate into it) from a device counter until the queue is empty.
+//     accumulate into it) from per-XCD device queues until all are empty.
 //     The inner loop is the dense engine GEMM's: double-buffered k-fast LDS
 //     tiles feeding 16x16x4 MFMAs (FP64 for R8, FP32 for R4). The epilogue
 //     scatters the tile into the D block through its index maps. Every D
@@ -78,7 +78,25 @@ __device__ __forceinline__ int32_t map_off32(const IdxMap &mp, uint32_t v)
   return o;
 }
 
-__global__ void k_zero_counter(int *c) { *c = 0; }
+__global__ void k_zero_counters(WorkQueues *q)
+{
+  for (int i = 0; i < kQueues; ++i) q->counter[i] = 0;
+}
+
+// The next work item for this workgroup (called by one thread): its XCD's
+// queue first, then the others in turn; -1 when every queue is drained.
+__device__ __forceinline__ int take_item(WorkQueues *q)
+{
+  const int home = (int)(blockIdx.x % kQueues);
+  for (int t = 0; t < kQueues; ++t) {
+    const int qq = (home + t) % kQueues;
+    const int len = q->end[qq] - q->begin[qq];
+    if (__hip_atomic_load(&q->counter[qq], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= len) continue;
+    const int v = atomicAdd(&q->counter[qq], 1);
+    if (v < len) return q->begin[qq] + v;
+  }
+  return -1;
+}
 
 // D blocks are reached through generic pointers stored in the work items;
 // accessed through a global-address-space pointer they compile to global
@@ -234,8 +252,8 @@ __global__ __launch_bounds__(256) void k_pack_tiled(const PackJob *__restrict__ 
 
 template <typename T, int OCC>
 __global__ __launch_bounds__(256, OCC) void k_grouped_gemm(const BlockOp *__restrict__ ops,
-                                                           const WorkItem *__restrict__ items, int nitems,
-                                                           int *__restrict__ counter, double alpha, int accumulate)
+                                                           const WorkItem *__restrict__ items,
+                                                           WorkQueues *__restrict__ queues, double alpha, int accumulate)
 {
   constexpr int BM = kGroupedTile, BN = kGroupedTile, BK = 16, KP = BK + (sizeof(T) == 4 ? 4 : 2);
   constexpr int FM = BM / 32, FN = BN / 32;
@@ -251,11 +269,13 @@ __global__ __launch_bounds__(256, OCC) void k_grouped_gemm(const BlockOp *__rest
   const int s_r = tid / (BK / LA), s_k = (tid % (BK / LA)) * LA;  // staging row, first k
   const int fr = lane & 15, fg = (lane >> 4) * 4, fk = lane >> 4;
   while (true) {
-    if (tid == 0) next_item = atomicAdd(counter, 1);
+    if (tid == 0) next_item = take_item(queues);
     __syncthreads();
-    const int it = next_item;
-    __syncthreads();
-    if (it >= nitems) break;
+    // uniform across the wave, so that the item's and its block products'
+    // descriptors are read with scalar loads
+    const int it = __builtin_amdgcn_readfirstlane(next_item);
+    __syncthreads();  // everyone has read next_item before it is replaced
+    if (it < 0) break;
     const WorkItem w = items[it];
     const int64_t m0 = (int64_t)w.mt * BM, n0 = (int64_t)w.nt * BN;
     AccT acc[FM][FN];
@@ -375,8 +395,8 @@ __global__ __launch_bounds__(256, OCC) void k_grouped_gemm(const BlockOp *__rest
 // read from its last band (those product rows and columns are not stored).
 constexpr int kB3GroupedThreads = 512;
 __global__ __launch_bounds__(kB3GroupedThreads) void k_grouped_b3(const BlockOp *__restrict__ ops,
-                                                                  const WorkItem *__restrict__ items, int nitems,
-                                                                  int *__restrict__ counter, double alpha,
+                                                                  const WorkItem *__restrict__ items,
+                                                                  WorkQueues *__restrict__ queues, double alpha,
                                                                   int accumulate)
 {
   constexpr int BM = kGroupedTile, BN = kGroupedTile, KB = 32, RB = 16;
@@ -396,11 +416,11 @@ __global__ __launch_bounds__(kB3GroupedThreads) void k_grouped_b3(const BlockOp 
   const int wm = (wave & 1) * WM, wn = ((wave >> 1) & 3) * WN;
   const int fr = lane & 15, fpos = PackLay::pos(fr, (lane >> 4) & 3) * 8;
   while (true) {
-    if (tid == 0) next_item = atomicAdd(counter, 1);
+    if (tid == 0) next_item = take_item(queues);
     __syncthreads();
     const int it = next_item;
     __syncthreads();
-    if (it >= nitems) break;
+    if (it < 0) break;
     const WorkItem w = items[it];
     const int64_t m0 = (int64_t)w.mt * BM, n0 = (int64_t)w.nt * BN;
     // k-stages of the whole chain
@@ -570,20 +590,20 @@ int launch_pack_tiled(int kind, bool split3, const PackJob *jobs, int njobs, int
   return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
 }
 
-int launch_grouped(int kind, bool split3, const BlockOp *ops, const WorkItem *items, int nitems, int *counter,
+int launch_grouped(int kind, bool split3, const BlockOp *ops, const WorkItem *items, WorkQueues *queues,
                    double alpha, bool accumulate, int grid, hipStream_t s)
 {
-  if (nitems <= 0) return 0;
-  hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, counter);
+  if (grid <= 0) return 0;
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, s, queues);
   if (split3) {
     if (kind != R4 && kind != C4) return TALSH_INVALID_ARGS;
-    hipLaunchKernelGGL(k_grouped_b3, dim3(grid), dim3(kB3GroupedThreads), 0, s, ops, items, nitems, counter, alpha,
+    hipLaunchKernelGGL(k_grouped_b3, dim3(grid), dim3(kB3GroupedThreads), 0, s, ops, items, queues, alpha,
                        accumulate ? 1 : 0);
   } else if (kind == R8 || kind == C8)
-    hipLaunchKernelGGL((k_grouped_gemm<double, 2>), dim3(grid), dim3(256), 0, s, ops, items, nitems, counter, alpha,
+    hipLaunchKernelGGL((k_grouped_gemm<double, 2>), dim3(grid), dim3(256), 0, s, ops, items, queues, alpha,
                        accumulate ? 1 : 0);
   else if (kind == R4 || kind == C4)
-    hipLaunchKernelGGL((k_grouped_gemm<float, 2>), dim3(grid), dim3(256), 0, s, ops, items, nitems, counter, alpha,
+    hipLaunchKernelGGL((k_grouped_gemm<float, 2>), dim3(grid), dim3(256), 0, s, ops, items, queues, alpha,
                        accumulate ? 1 : 0);
   else
     return TALSH_INVALID_ARGS;
