@@ -443,8 +443,8 @@ bool resident_on(const talsh_tens_t *t, int flat)
   return false;
 }
 
-// Exhaustive autotuner: every (matricization, tile) candidate of the
-// contraction runs once as a warm-up and once or twice timed (kernel time of
+// Exhaustive autotuner: every (matricization or fused gather, tile) candidate
+// of the contraction runs once as a warm-up and once or twice timed (kernel time of
 // its transposes and matrix product, HIP events on the launch stream) on the
 // caller's L and R images into a scratch D on the executing GPU; the fastest
 // is memoized under key. Blocking. false when it cannot tune now (L or R not
@@ -466,7 +466,8 @@ bool autotune_choose(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens
   }
   if (!resident_on(ltens, d->flat_id) || !resident_on(rtens, d->flat_id)) return false;
   std::vector<PlanCandidate> cands;
-  if (plan_candidates(dig, dr, lr, rr, dd, ld, rd, kind, d->r4mode, acc, d->num_cus, cands) || cands.empty())
+  if (plan_candidates(dig, dr, lr, rr, dd, ld, rd, kind, d->r4mode, acc, d->num_cus, cands, true) ||
+      cands.empty())
     return false;
   talsh_tens_t scratch;
   talshTensorClean(&scratch);
@@ -1065,7 +1066,10 @@ static int contract_impl(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *l
         if (plan_tile < 0) plan_tile = t;
       }
     }
-    const int e = plan_contraction(dig, dr, lr, rr, dd, ld, rd, es, acc, plan_variant, P, nullptr);
+    int e = plan_contraction(dig, dr, lr, rr, dd, ld, rd, es, acc, plan_variant, P, nullptr);
+    // the three-way BF16 split reads matricized operands: no fused gather
+    if (!e && P.fused && (kind == R4 || kind == C4) && d->r4mode == R4Mode::kBF16x3)
+      e = plan_contraction(dig, dr, lr, rr, dd, ld, rd, es, acc, -1, P, nullptr);
     if (e) return e;
   }
   size_t ll = 1, lrr = 1, lc = 1;
@@ -1123,6 +1127,10 @@ static int contract_impl(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *l
         return 0;
       }
       int e = 0;
+      if (P.fused) {
+        wsb = gemm_gather_workspace_bytes(gemm_args(t.d->r4mode));
+        return wsb ? t.alloc(wsb, &ws) : 0;
+      }
       if (!P.l_o2n.empty()) e = t.alloc(tensor_volume(L->t) * es, &lt);
       if (!e && !P.r_o2n.empty()) e = t.alloc(tensor_volume(R->t) * es, &rt);
       if (!e && !P.d_in_o2n.empty()) e = t.alloc(tensor_volume(D->t) * es, &dt);
@@ -1166,6 +1174,19 @@ static int contract_impl(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *l
         }
         return launch_dot(D->exec, L->exec, rsrc, kind, tensor_volume(L->t), are, aim, lconj, rconj, acc, scr,
                           1024 * sizeof(double2), s);
+      }
+      if (P.fused) {  // one product straight over the tensors
+        GemmArgs g = gemm_args(dev->r4mode);
+        g.a = P.swap ? R->exec : L->exec;
+        g.b = P.swap ? L->exec : R->exec;
+        g.c = D->exec;
+        if (k->timed) (void)hipEventRecord(k->ev_mm0, s);
+        prof_begin(s, kKGemm, pr);
+        const int e = launch_gemm_gather(g, P.maps, ws, wsb, s);
+        prof_end(s, kKGemm, pr);
+        if (k->timed) (void)hipEventRecord(k->ev_mm1, s);
+        k->has_mm = k->timed;
+        return e;
       }
       // GEMM path
       const void *la = L->exec, *rb = R->exec;
@@ -1744,6 +1765,21 @@ int talshAmdPlanCandidates(const char *cptrn, int drank, const int *ddims, int l
   *ncand = (int)all.size();
   for (int i = 0; i < (int)all.size() && i < cap; ++i)
     if (moved) moved[i] = all[i].bytes_moved;
+  return 0;
+}
+
+int talshAmdPlanFusedCandidates(const char *cptrn, int drank, const int *ddims, int lrank, const int *ldims,
+                                int rrank, const int *rdims, int *nfused)
+{
+  if (!cptrn || !nfused) return TALSH_INVALID_ARGS;
+  int digc[2 * MAX_TENSOR_RANK], dr, lr, rr, cj;
+  if (talsh_get_contr_ptrn_str2dig(cptrn, digc, &dr, &lr, &rr, &cj)) return TALSH_INVALID_ARGS;
+  if (dr != drank || lr != lrank || rr != rrank || dr == 0 || lr == 0 || rr == 0) return TALSH_INVALID_ARGS;
+  std::vector<int> dig(digc, digc + lr + rr), dd(ddims, ddims + dr), ld(ldims, ldims + lr), rd(rdims, rdims + rr);
+  std::vector<ContractPlan> fz;
+  const int e = plan_fused(dig, dr, lr, rr, dd, ld, rd, fz);
+  if (e) return e;
+  *nfused = (int)fz.size();
   return 0;
 }
 

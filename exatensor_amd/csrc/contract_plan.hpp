@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "kernels.hpp"
 #include "runtime.hpp"
 
 namespace xt {
@@ -21,7 +22,22 @@ struct ContractPlan {
   bool a_kfast = true, b_kfast = true;
   size_t m = 1, n = 1, k = 1;
   double bytes_moved = 0;
+  // fused index gather: no transposes, the product reads L and R and updates
+  // D in place through offset tables (maps in the launched orientation)
+  bool fused = false;
+  GatherMaps maps;
 };
+
+// Plan variants from kFusedVariantBase on are the fused-gather candidates
+// (kFusedVariantBase + i: candidate i of plan_fused).
+constexpr int kFusedVariantBase = 1000;
+
+// Fused-gather candidates of a GEMM-shaped contraction: contracted indices in
+// L's or in R's order, D or D^T as the product's result; none when an index
+// group has more than kGatherMaxDims dimensions after unit extents are
+// dropped and contiguous runs merged.
+int plan_fused(const std::vector<int> &dig, int dr, int lr, int rr, const std::vector<int> &dd,
+               const std::vector<int> &ld, const std::vector<int> &rd, std::vector<ContractPlan> &out);
 
 std::vector<int> reorder(const std::vector<int> &from, const std::vector<int> &to);
 bool is_identity(const std::vector<int> &p);
@@ -29,7 +45,8 @@ bool is_identity(const std::vector<int> &p);
 // Distinct matricization candidates of a GEMM-shaped contraction (digital
 // pattern dig, extents dd/ld/rd), ranked by the byte-count heuristic; P is
 // candidate `variant` (the heuristic's choice for variant < 0 or out of
-// range). Hyper-indices (in L, R and D) are rejected with TALSH_INVALID_ARGS.
+// range; variant >= kFusedVariantBase picks a fused-gather candidate). Hyper-
+// indices (in L, R and D) are rejected with TALSH_INVALID_ARGS.
 int plan_contraction(const std::vector<int> &dig, int dr, int lr, int rr, const std::vector<int> &dd,
                      const std::vector<int> &ld, const std::vector<int> &rd, int es, bool acc, int variant,
                      ContractPlan &P, std::vector<ContractPlan> *all);
@@ -44,9 +61,11 @@ struct PlanCandidate {
 };
 
 // Every candidate of a GEMM-shaped contraction with its features.
+// with_fused adds the fused-gather candidates (native precision only) after
+// the matricization candidates; the learned selector ranks only the latter.
 int plan_candidates(const std::vector<int> &dig, int dr, int lr, int rr, const std::vector<int> &dd,
                     const std::vector<int> &ld, const std::vector<int> &rd, int kind, R4Mode r4mode, bool acc,
-                    int num_cus, std::vector<PlanCandidate> &out);
+                    int num_cus, std::vector<PlanCandidate> &out, bool with_fused = false);
 
 // Training domain of the selector (random contractions up to 2^35 flops and
 // operands of up to 2^26 elements, with margin); larger problems use the

@@ -14,6 +14,7 @@
 // C is m x n column-major. Complex operands are interleaved (re, im) and are
 // split into real/imaginary LDS planes while staging.
 #include <algorithm>
+#include <type_traits>
 
 #include "bf16x3.hpp"
 #include "kernels.hpp"
@@ -57,6 +58,15 @@ struct GemmParams {
   // part == nullptr: one pass over all of K, epilogue into C
   void *part;
   int64_t kchunk;
+  // fused index gather (GATHER kernels): A element (r, kk) at a_row[r] +
+  // a_kof[kk], B element (c, kk) at b_row[c] + b_kof[kk], C element (r, c) at
+  // c_row[r] + c_col[c] (element offsets into the tensor bodies); vec_a /
+  // vec_b: aligned groups of 16 bytes along the staging direction are
+  // contiguous
+  const int64_t *a_row, *a_kof, *b_row, *b_kof, *c_row, *c_col;
+  int64_t a_ks0, b_ks0;      // stride of the first contracted dimension in A / B (gather)
+  int64_t a_rs0, b_rs0;      // stride of the first row dimension of A / B (gather)
+  const void *zpad;          // 64 zero bytes, 16-byte aligned (gather)
 };
 
 // Maps a workgroup to an output tile: first gather the workgroups that the
@@ -93,7 +103,11 @@ struct Frag4<float> { typedef float4 type; };
 template <>
 struct Frag4<double> { typedef double4 type; };
 
-template <typename T, bool CPLX, bool AK, bool BKF, int BM, int BN, int BK, int OCC>
+// GATHER: 0 matricized operands; 1 fused gather where every thread's staged
+// elements stay inside one run of the staging direction's first dimension
+// (offsets by stride arithmetic from one table entry); 2 fused gather with a
+// table lookup per staged element or 16-byte group.
+template <typename T, bool CPLX, bool AK, bool BKF, int BM, int BN, int BK, int OCC, int GATHER = 0>
 __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
 {
   constexpr int NP = CPLX ? 2 : 1;                 // LDS planes (re, im)
@@ -126,8 +140,112 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
   if (BKF) { b_r = tid / (BK / LB); b_k = (tid % (BK / LB)) * LB; }
   else { b_k = tid / (BN / LB); b_r = (tid % (BN / LB)) * LB; }
 
+  // fused gather: offset of the thread's first staged row (loop invariant)
+  int64_t ga_row = 0, gb_row = 0;
+  if (GATHER) {
+    ga_row = p.a_row[m0 + a_r < p.m ? m0 + a_r : 0];
+    gb_row = p.b_row[n0 + b_r < p.n ? n0 + b_r : 0];
+  }
+  // One operand element (or 16-byte group) of a gather load; lanes out of
+  // range (ok == false) read a zero block instead. The load itself is
+  // unconditional and its value is used as loaded: a lane-masked load, or a
+  // select on the loaded value, makes the compiler wait for the loads in
+  // flight right there instead of at the LDS store.
+  constexpr int VW = 16 / (int)sizeof(T);
+  typedef T TV __attribute__((ext_vector_type(16 / sizeof(T))));
+  const T *zpad = static_cast<const T *>(p.zpad);
+  auto gather_vec = [&](const T *X, int64_t off, bool ok, T *dst) {
+    const TV v = *reinterpret_cast<const TV *>(ok ? X + off : zpad);
+#pragma unroll
+    for (int e = 0; e < VW; ++e) dst[e] = v[e];
+  };
+  auto gather_one = [&](const T *X, int64_t off, bool ok, int conj, T &re, T &im) {
+    if (CPLX) {
+      const T *q = ok ? X + 2 * off : zpad;
+      re = q[0];
+      const T b = q[1];
+      im = conj ? -b : b;
+    } else {
+      re = *(ok ? X + off : zpad);
+    }
+  };
+  // Gather staging of one operand tile, kf: along k (row offset fixed), else
+  // along rows (one k per thread). kbase is the table entry of the thread's
+  // first k (loaded one tile ahead), row_base the offset of its first row.
+  // GATHER == 1: a thread's L consecutive k (rows) stay inside one run of the
+  // first contracted (row) dimension, stride ks0 (rs0) (the launcher checks
+  // that the extent is a multiple of L); GATHER == 2: each k (row) looks up
+  // its own table entry.
+  auto gather_load = [&](const T *X, const int64_t *rowtab, int64_t row_base, int64_t rs0, const int64_t *kof,
+                         int64_t kbase, int64_t ks0, int64_t rows, int64_t row0,
+                         int k_off, int64_t k0, bool vec, int conj, auto kf, auto dst) {
+    constexpr int L = (int)(sizeof(dst[0]) / sizeof(T));
+    constexpr bool RUNS = GATHER == 1;
+    const int64_t kk0 = k0 + k_off;
+    if (decltype(kf)::value) {
+      const bool rok = row0 < rows;
+      if (RUNS) {
+        if (!CPLX && vec) {
+#pragma unroll
+          for (int i = 0; i < L; i += VW) gather_vec(X, row_base + kbase + i, rok && kk0 + i < kend, &dst[0][i]);
+        } else {
+#pragma unroll
+          for (int i = 0; i < L; ++i)
+            gather_one(X, row_base + kbase + i * ks0, rok && kk0 + i < kend, conj, dst[0][i], dst[1 % NP][i]);
+        }
+      } else if (!CPLX && vec) {
+#pragma unroll
+        for (int i = 0; i < L; i += VW) {
+          const bool ok = rok && kk0 + i < kend;
+          gather_vec(X, row_base + kof[ok ? kk0 + i : 0], ok, &dst[0][i]);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < L; ++i) {
+          const bool ok = rok && kk0 + i < kend;
+          gather_one(X, row_base + kof[ok ? kk0 + i : 0], ok, conj, dst[0][i], dst[1 % NP][i]);
+        }
+      }
+    } else {
+      const bool kok = kk0 < kend;
+      if (RUNS) {
+        if (!CPLX && vec) {
+#pragma unroll
+          for (int i = 0; i < L; i += VW) gather_vec(X, row_base + i + kbase, kok && row0 + i < rows, &dst[0][i]);
+        } else {
+#pragma unroll
+          for (int i = 0; i < L; ++i)
+            gather_one(X, row_base + i * rs0 + kbase, kok && row0 + i < rows, conj, dst[0][i], dst[1 % NP][i]);
+        }
+      } else if (!CPLX && vec) {
+#pragma unroll
+        for (int i = 0; i < L; i += VW) {
+          const bool ok = kok && row0 + i < rows;
+          gather_vec(X, rowtab[ok ? row0 + i : 0] + kbase, ok, &dst[0][i]);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < L; ++i) {
+          const bool ok = kok && row0 + i < rows;
+          gather_one(X, rowtab[ok ? row0 + i : 0] + kbase, ok, conj, dst[0][i], dst[1 % NP][i]);
+        }
+      }
+    }
+  };
+  // table entry of a thread's first k in the tile at k0 (clamped into range)
+  auto kof_at = [&](const int64_t *kof, int64_t k0, int k_off) -> int64_t {
+    const int64_t kk = k0 + k_off;
+    return kof[kk < kend ? kk : kbeg];
+  };
+  int64_t ka_next = 0, kb_next = 0;  // gather: entries of the next tile to load
+
   T ra[NP][LA], rb[NP][LB];
   auto load_a = [&](int64_t k0) {
+    if (GATHER) {
+      gather_load(A, p.a_row, ga_row, p.a_rs0, p.a_kof, ka_next, p.a_ks0, p.m, m0 + a_r, a_k, k0,
+                  p.vec_a != 0, p.conj_a, std::integral_constant<bool, AK>(), ra);
+      return;
+    }
     const bool vec = !CPLX && AK && interior_mn && (k0 + BK <= kend) && p.vec_a;
     if (vec) {
       const T *src = A + (m0 + a_r) * p.lda + k0 + a_k;
@@ -150,6 +268,11 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
     }
   };
   auto load_b = [&](int64_t k0) {
+    if (GATHER) {
+      gather_load(B, p.b_row, gb_row, p.b_rs0, p.b_kof, kb_next, p.b_ks0, p.n, n0 + b_r, b_k, k0,
+                  p.vec_b != 0, p.conj_b, std::integral_constant<bool, BKF>(), rb);
+      return;
+    }
     const bool vec = !CPLX && BKF && interior_mn && (k0 + BK <= kend) && p.vec_b;
     if (vec) {
       const T *src = B + (n0 + b_r) * p.ldb + k0 + b_k;
@@ -202,8 +325,16 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
       for (int j = 0; j < FN; ++j) acc[q][i][j] = AccT{0, 0, 0, 0};
 
   const int64_t nk = (kend - kbeg + BK - 1) / BK;
+  if (GATHER) {
+    ka_next = kof_at(p.a_kof, kbeg, a_k);
+    kb_next = kof_at(p.b_kof, kbeg, b_k);
+  }
   load_a(kbeg);
   load_b(kbeg);
+  if (GATHER) {  // table entries one tile ahead of the operand loads
+    ka_next = kof_at(p.a_kof, kbeg + BK, a_k);
+    kb_next = kof_at(p.b_kof, kbeg + BK, b_k);
+  }
   store(0);
   __syncthreads();
   const int fr = lane & 15, fg = (lane >> 4) * 4;
@@ -212,6 +343,10 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
     if (kt + 1 < nk) {
       load_a(kbeg + (kt + 1) * BK);
       load_b(kbeg + (kt + 1) * BK);
+      if (GATHER) {
+        ka_next = kof_at(p.a_kof, kbeg + (kt + 2) * BK, a_k);
+        kb_next = kof_at(p.b_kof, kbeg + (kt + 2) * BK, b_k);
+      }
     }
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 16) {
@@ -255,6 +390,14 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
   T *Pz = split ? static_cast<T *>(p.part) + (int64_t)blockIdx.y * p.m * p.n : nullptr;
   const T ar = (T)p.alpha_re, ai = (T)p.alpha_im;
   const int fk = (lane >> 4) & 3;
+  int64_t gcol[FN];
+  if (GATHER && !split) {
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int64_t col = n0 + wn + j * 16 + fr;
+      gcol[j] = p.c_col[col < p.n ? col : 0];
+    }
+  }
   // Per 16-row block of the wave: the offsets (-1: outside C), then every C
   // load of the block at once (clamped addresses, no lane-masked branches),
   // one wait, then the stores; a load-add-store per element would wait for
@@ -263,15 +406,25 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
   for (int i = 0; i < FM; ++i) {
     int64_t off[FN][4];
     T cv[FN][4][NP];
+    // accumulator layout: FP32 16x16x4 keeps rows 4*(lane/16)+r in a lane,
+    // FP64 16x16x4 keeps rows lane/16 + 4*r
+    int64_t grow[4];
+    if (GATHER && !split) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm + i * 16 + (sizeof(T) == 8 ? fk + 4 * r : fk * 4 + r);
+        grow[r] = p.c_row[row < p.m ? row : 0];
+      }
+    }
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       const int64_t col = n0 + wn + j * 16 + fr;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        // accumulator layout: FP32 16x16x4 keeps rows 4*(lane/16)+r in a
-        // lane, FP64 16x16x4 keeps rows lane/16 + 4*r
         const int64_t row = m0 + wm + i * 16 + (sizeof(T) == 8 ? fk + 4 * r : fk * 4 + r);
-        off[j][r] = (row < p.m && col < p.n) ? (split ? row + col * p.m : row + col * p.ldc) : -1;
+        off[j][r] = (row < p.m && col < p.n)
+                        ? (split ? row + col * p.m : GATHER ? grow[r] + gcol[j] : row + col * p.ldc)
+                        : -1;
       }
     }
     if (!split && !p.beta_zero) {
@@ -763,15 +916,67 @@ __global__ __launch_bounds__(256) void k_splitk_sum(const T *__restrict__ part, 
   }
 }
 
-template <typename T, bool CPLX, int BM, int BN, int BK, int OCC>
+// Split-K sum of a gathered product: the partials are m x n column-major,
+// C element (r, c) sits at c_row[r] + c_col[c].
+template <typename T>
+__global__ __launch_bounds__(256) void k_splitk_sum_gather(const T *__restrict__ part, int nz, int64_t m, int64_t mn,
+                                                           const int64_t *__restrict__ c_row,
+                                                           const int64_t *__restrict__ c_col, T *__restrict__ c,
+                                                           double alpha, int beta_zero)
+{
+  const int64_t st = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < mn; i += st) {
+    const int64_t col = i / m, row = i - col * m;
+    const int64_t off = c_row[row] + c_col[col];
+    double v = 0.0;
+    for (int z = 0; z < nz; ++z) v += (double)part[(int64_t)z * mn + i];
+    v *= alpha;
+    if (!beta_zero) v += (double)c[off];
+    c[off] = (T)v;
+  }
+}
+
+// Offset tables of a gathered product, back to back: table t holds
+// count[t] = prod(ext[t][.]) entries, entry i = sum_j digit_j(i) * str[t][j].
+// After the tables, from entry zero0 on, kGatherZeroWords zero entries (the
+// zero block of out-of-range gather loads).
+constexpr int kGatherZeroWords = 8;
+struct GatherTables {
+  int64_t *out;
+  int64_t start[7];  // first entry of table t; start[6] = total
+  int64_t zero0;     // first zero entry (even, so the block is 16-byte aligned)
+  int nd[6];
+  int ext[6][kGatherMaxDims];
+  int64_t str[6][kGatherMaxDims];
+};
+
+__global__ __launch_bounds__(256) void k_gather_tables(const GatherTables g)
+{
+  const int64_t st = (int64_t)gridDim.x * 256;
+  if (blockIdx.x == 0 && threadIdx.x < kGatherZeroWords) g.out[g.zero0 + threadIdx.x] = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < g.start[6]; i += st) {
+    int t = 0;
+    while (t < 5 && i >= g.start[t + 1]) ++t;
+    int64_t x = i - g.start[t], off = 0;
+    for (int j = 0; j < g.nd[t]; ++j) {
+      const int64_t e = g.ext[t][j];
+      const int64_t q = x / e;
+      off += (x - q * e) * g.str[t][j];
+      x = q;
+    }
+    g.out[i] = off;
+  }
+}
+
+template <typename T, bool CPLX, int BM, int BN, int BK, int OCC, int GATHER = 0>
 int launch_family(GemmParams p, bool ak, bool bk, hipStream_t s)
 {
   p.tiles_m = (int)((p.m + BM - 1) / BM);
   p.tiles_n = (int)((p.n + BN - 1) / BN);
-  if (ak && bk) return launch_grid(k_gemm_mfma<T, CPLX, true, true, BM, BN, BK, OCC>, p, s);
-  if (ak) return launch_grid(k_gemm_mfma<T, CPLX, true, false, BM, BN, BK, OCC>, p, s);
-  if (bk) return launch_grid(k_gemm_mfma<T, CPLX, false, true, BM, BN, BK, OCC>, p, s);
-  return launch_grid(k_gemm_mfma<T, CPLX, false, false, BM, BN, BK, OCC>, p, s);
+  if (ak && bk) return launch_grid(k_gemm_mfma<T, CPLX, true, true, BM, BN, BK, OCC, GATHER>, p, s);
+  if (ak) return launch_grid(k_gemm_mfma<T, CPLX, true, false, BM, BN, BK, OCC, GATHER>, p, s);
+  if (bk) return launch_grid(k_gemm_mfma<T, CPLX, false, true, BM, BN, BK, OCC, GATHER>, p, s);
+  return launch_grid(k_gemm_mfma<T, CPLX, false, false, BM, BN, BK, OCC, GATHER>, p, s);
 }
 
 constexpr int kB3Form = 16;               // kernel form of the three-way split
@@ -1084,6 +1289,151 @@ int launch_gemm_ws(const GemmArgs &g, void *ws, size_t ws_bytes, hipStream_t s)
     return launch_family<float, true, 64, 64, 16, 2>(p, g.a_kfast, g.b_kfast, s);
   default: return TALSH_INVALID_ARGS;
   }
+}
+
+namespace {
+
+int64_t group_count(const GatherGroup &g)
+{
+  int64_t v = 1;
+  for (int j = 0; j < g.nd; ++j) v *= g.ext[j];
+  return v;
+}
+
+// 16-byte groups along a staging direction are contiguous and aligned when the
+// group's first dimension is the tensor's stride-1 dimension with an extent
+// that is a multiple of the vector width (every other stride of the tensor is
+// then a multiple of that extent) and the body is 16-byte aligned.
+bool gather_vec_ok(const GatherGroup &g, int es, const void *base)
+{
+  return g.nd > 0 && g.str[0] == 1 && (g.ext[0] * es) % 16 == 0 && ((uintptr_t)base % 16) == 0;
+}
+
+size_t gather_tables_bytes(const GemmArgs &g)
+{
+  return ((2 * (g.m + g.n + g.k) + 2 + kGatherZeroWords) * sizeof(int64_t) + 255) / 256 * 256;
+}
+
+}  // namespace
+
+size_t gemm_gather_workspace_bytes(const GemmArgs &g)
+{
+  if (gemm_uses_split3(g)) return 0;
+  const size_t tb = gather_tables_bytes(g);
+  if (g.kind == C8 || g.kind == C4) return tb;
+  const int tile = g.tile < 0 ? gemm_default_tile(g.kind, g.r4mode, g.m, g.n, g.k) : g.tile;
+  int64_t kc;
+  const int nz = splitk_parts(g, tile, &kc);
+  return tb + (nz > 1 ? (size_t)nz * g.m * g.n * (size_t)kind_size(g.kind) + 16 : 0);
+}
+
+int launch_gemm_gather(const GemmArgs &g, const GatherMaps &mp, void *ws, size_t ws_bytes, hipStream_t s)
+{
+  if (gemm_uses_split3(g)) return TALSH_NOT_IMPLEMENTED;
+  if (g.m == 0 || g.n == 0) return 0;
+  if (g.k == 0 && !g.beta_zero) return 0;  // empty contraction leaves C unchanged
+  const GatherGroup *grp[6] = {&mp.a_row, &mp.a_k, &mp.b_row, &mp.b_k, &mp.c_row, &mp.c_col};
+  const int64_t want[6] = {(int64_t)g.m, (int64_t)g.k, (int64_t)g.n, (int64_t)g.k, (int64_t)g.m, (int64_t)g.n};
+  GatherTables T{};
+  T.start[0] = 0;
+  for (int t = 0; t < 6; ++t) {
+    if (grp[t]->nd < 0 || grp[t]->nd > kGatherMaxDims || group_count(*grp[t]) != want[t]) return TALSH_INVALID_ARGS;
+    T.nd[t] = grp[t]->nd;
+    for (int j = 0; j < grp[t]->nd; ++j) {
+      if (grp[t]->ext[j] < 1 || grp[t]->str[j] < 0) return TALSH_INVALID_ARGS;
+      T.ext[t][j] = grp[t]->ext[j];
+      T.str[t][j] = grp[t]->str[j];
+    }
+    T.start[t + 1] = T.start[t] + want[t];
+  }
+  const size_t tb = gather_tables_bytes(g);
+  if (!ws || ws_bytes < tb) return TALSH_INVALID_ARGS;
+  if ((int64_t)((g.m + 63) / 64) * (int64_t)((g.n + 63) / 64) >= (1ll << 31)) return TALSH_LIMIT_EXCEEDED;
+  int64_t *tab = static_cast<int64_t *>(ws);
+  T.out = tab;
+  T.zero0 = (T.start[6] + 1) / 2 * 2;
+  hipLaunchKernelGGL(k_gather_tables, dim3((unsigned)std::min<int64_t>((T.start[6] + 255) / 256, 1024)), dim3(256), 0,
+                     s, T);
+  if (!XT_HIP_OK(hipGetLastError())) return TALSH_FAILURE;
+
+  const int es = kind_size(g.kind);
+  const bool cplx = g.kind == C8 || g.kind == C4;
+  GemmParams p{};
+  p.a = g.a; p.b = g.b; p.c = g.c;
+  p.m = (int64_t)g.m; p.n = (int64_t)g.n; p.k = (int64_t)g.k;
+  p.alpha_re = g.alpha_re; p.alpha_im = g.alpha_im;
+  p.beta_zero = g.beta_zero ? 1 : 0;
+  p.conj_a = g.conj_a; p.conj_b = g.conj_b;
+  p.xcd_remap = 1;
+  p.vec_a = !cplx && gather_vec_ok(g.a_kfast ? mp.a_k : mp.a_row, es, g.a);
+  p.vec_b = !cplx && gather_vec_ok(g.b_kfast ? mp.b_k : mp.b_row, es, g.b);
+  p.a_row = tab + T.start[0];
+  p.a_kof = tab + T.start[1];
+  p.b_row = tab + T.start[2];
+  p.b_kof = tab + T.start[3];
+  p.c_row = tab + T.start[4];
+  p.c_col = tab + T.start[5];
+  p.zpad = tab + T.zero0;
+  p.a_ks0 = mp.a_k.nd ? mp.a_k.str[0] : 0;
+  p.b_ks0 = mp.b_k.nd ? mp.b_k.str[0] : 0;
+  p.a_rs0 = mp.a_row.nd ? mp.a_row.str[0] : 0;
+  p.b_rs0 = mp.b_row.nd ? mp.b_row.str[0] : 0;
+  // every thread's staged run (8 elements on 128x128 tiles, 4 on 64x64) stays
+  // inside one run of the staging direction's first dimension
+  auto runs = [&](int L) {
+    const GatherGroup &sa = g.a_kfast ? mp.a_k : mp.a_row, &sb = g.b_kfast ? mp.b_k : mp.b_row;
+    return (sa.nd ? sa.ext[0] : 1) % L == 0 && (sb.nd ? sb.ext[0] : 1) % L == 0;
+  };
+  const int tile = g.tile < 0 ? gemm_default_tile(g.kind, g.r4mode, g.m, g.n, g.k) : g.tile;
+  if (!cplx) {
+    int64_t kc;
+    const int nz = splitk_parts(g, tile, &kc);
+    const int64_t mn = (int64_t)g.m * (int64_t)g.n;
+    if (nz > 1 && ws_bytes >= tb + (size_t)nz * (size_t)mn * es) {
+      p.part = static_cast<char *>(ws) + tb;
+      p.kchunk = kc;
+      const unsigned rb = (unsigned)std::min<int64_t>((mn + 255) / 256, 4096);
+      int e;
+      if (g.kind == R8) {
+        e = runs(4) ? launch_family<double, false, 64, 64, 16, 2, 1>(p, g.a_kfast, g.b_kfast, s)
+                    : launch_family<double, false, 64, 64, 16, 2, 2>(p, g.a_kfast, g.b_kfast, s);
+        if (!e)
+          hipLaunchKernelGGL(k_splitk_sum_gather<double>, dim3(rb), dim3(256), 0, s,
+                             static_cast<const double *>(p.part), nz, p.m, mn, p.c_row, p.c_col,
+                             static_cast<double *>(g.c), g.alpha_re, p.beta_zero);
+      } else {
+        e = runs(4) ? launch_family<float, false, 64, 64, 16, 2, 1>(p, g.a_kfast, g.b_kfast, s)
+                    : launch_family<float, false, 64, 64, 16, 2, 2>(p, g.a_kfast, g.b_kfast, s);
+        if (!e)
+          hipLaunchKernelGGL(k_splitk_sum_gather<float>, dim3(rb), dim3(256), 0, s,
+                             static_cast<const float *>(p.part), nz, p.m, mn, p.c_row, p.c_col,
+                             static_cast<float *>(g.c), g.alpha_re, p.beta_zero);
+      }
+      if (e) return e;
+      return XT_HIP_OK(hipGetLastError()) ? 0 : TALSH_FAILURE;
+    }
+  }
+#define XT_GATHER(T_, CPLX_, BM_, OCC_)                                                          \
+  (runs(BM_ / 16) ? launch_family<T_, CPLX_, BM_, BM_, 16, OCC_, 1>(p, g.a_kfast, g.b_kfast, s) \
+                  : launch_family<T_, CPLX_, BM_, BM_, 16, OCC_, 2>(p, g.a_kfast, g.b_kfast, s))
+  switch (g.kind) {
+  case R8:
+    switch (tile) {
+    case 1: return XT_GATHER(double, false, 128, 1);
+    case 2: return XT_GATHER(double, false, 64, 2);
+    default: return XT_GATHER(double, false, 128, 2);
+    }
+  case C8: return XT_GATHER(double, true, 64, 2);
+  case R4:
+    switch (tile) {
+    case 1: return XT_GATHER(float, false, 128, 1);
+    case 2: return XT_GATHER(float, false, 64, 2);
+    default: return XT_GATHER(float, false, 128, 2);
+    }
+  case C4: return XT_GATHER(float, true, 64, 2);
+  default: return TALSH_INVALID_ARGS;
+  }
+#undef XT_GATHER
 }
 
 }  // namespace xt

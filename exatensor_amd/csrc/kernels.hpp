@@ -69,6 +69,27 @@ size_t gemm_workspace_bytes(const GemmArgs &g);
 // ws holds ws_bytes; a workspace smaller than gemm_workspace_bytes(g) is an
 // error for the three-way split and runs a real product unsplit.
 int launch_gemm_ws(const GemmArgs &g, void *ws, size_t ws_bytes, hipStream_t s);
+// Fused index gather: a contraction's product read straight from the
+// tensors, no transposes. Each matrix index (rows and k of A and of B, rows
+// and columns of C) is a group of tensor dimensions; element i of a group
+// sits at sum_j digit_j(i) * str[j] with the digits of i over ext[] (first
+// fastest). GemmArgs::a, b and c are the tensor bodies; a_kfast / b_kfast
+// name the staging direction of A and B (along k, or along rows), the one
+// that holds the tensor's stride-1 dimension. Native precision only (not the
+// three-way split).
+constexpr int kGatherMaxDims = 16;
+struct GatherGroup {
+  int nd = 0;
+  int ext[kGatherMaxDims] = {};
+  int64_t str[kGatherMaxDims] = {};
+};
+struct GatherMaps {
+  GatherGroup a_row, a_k, b_row, b_k, c_row, c_col;
+};
+// Workspace of launch_gemm_gather: the six offset tables (8 bytes per index)
+// and, for split K, the partial products.
+size_t gemm_gather_workspace_bytes(const GemmArgs &g);
+int launch_gemm_gather(const GemmArgs &g, const GatherMaps &maps, void *ws, size_t ws_bytes, hipStream_t s);
 // Fixed per-GPU scratch of the allocation-free product entry point: a
 // file-scope device array in k_gemm.hip (64 MiB, held by every GPU the
 // library runs on until the process ends). gemm_noalloc_scratch returns its

@@ -132,8 +132,126 @@ int plan_contraction(const std::vector<int> &dig, int dr, int lr, int rr, const 
   }
   cands.swap(uniq);
   if (all) *all = cands;
+  if (variant >= kFusedVariantBase) {
+    std::vector<ContractPlan> fz;
+    const int e = plan_fused(dig, dr, lr, rr, dd, ld, rd, fz);
+    if (e) return e;
+    if (variant - kFusedVariantBase < (int)fz.size()) {
+      P = fz[variant - kFusedVariantBase];
+      return 0;
+    }
+  }
   if (variant >= 0 && variant < (int)cands.size()) P = cands[variant];
   else P = cands.front();
+  return 0;
+}
+
+namespace {
+
+// Index group of one tensor for the fused gather: the labels' extents and
+// strides in that tensor, unit extents dropped and runs that stay contiguous
+// (stride[j + 1] == stride[j] * ext[j]) merged. false if more than
+// kGatherMaxDims dimensions remain.
+bool gather_group(const std::vector<int> &labs, const std::vector<int> &ext, const std::vector<int64_t> &str,
+                  GatherGroup &g)
+{
+  g = GatherGroup{};
+  std::vector<int64_t> e, s;
+  for (int lab : labs) {
+    if (ext[lab] == 1) continue;
+    if (!e.empty() && str[lab] == s.back() * e.back()) {
+      e.back() *= ext[lab];
+      continue;
+    }
+    e.push_back(ext[lab]);
+    s.push_back(str[lab]);
+  }
+  if ((int)e.size() > kGatherMaxDims) return false;
+  for (size_t j = 0; j < e.size(); ++j) {
+    if (e[j] > INT32_MAX) return false;
+    g.ext[j] = (int)e[j];
+    g.str[j] = s[j];
+  }
+  g.nd = (int)e.size();
+  return true;
+}
+
+}  // namespace
+
+int plan_fused(const std::vector<int> &dig, int dr, int lr, int rr, const std::vector<int> &dd,
+               const std::vector<int> &ld, const std::vector<int> &rd, std::vector<ContractPlan> &out)
+{
+  out.clear();
+  // labels: D index q -> q + 1; contracted index i of L -> 101 + i
+  std::vector<int> ext(256, 1);
+  std::vector<int64_t> sl(256, 0), sr(256, 0), sd(256, 0);
+  std::vector<int> llab(lr), rlab(rr), Kl, Kr, M, N;
+  int64_t st = 1;
+  for (int q = 0; q < dr; ++q) { ext[q + 1] = dd[q]; sd[q + 1] = st; st *= dd[q]; }
+  st = 1;
+  for (int i = 0; i < lr; ++i) {
+    llab[i] = dig[i] > 0 ? dig[i] : 100 + i + 1;
+    ext[llab[i]] = ld[i];
+    sl[llab[i]] = st;
+    st *= ld[i];
+    (dig[i] < 0 ? Kl : M).push_back(llab[i]);
+  }
+  st = 1;
+  for (int j = 0; j < rr; ++j) {
+    rlab[j] = dig[lr + j] > 0 ? dig[lr + j] : 100 + (-dig[lr + j]);
+    sr[rlab[j]] = st;
+    st *= rd[j];
+    (dig[lr + j] < 0 ? Kr : N).push_back(rlab[j]);
+  }
+  for (int lab : M)
+    if (std::find(N.begin(), N.end(), lab) != N.end()) return TALSH_INVALID_ARGS;  // hyper-index
+  auto vol = [&](const std::vector<int> &labs) {
+    size_t v = 1;
+    for (int l : labs) v *= (size_t)ext[l];
+    return v;
+  };
+  // the tensor's stride-1 label (first non-unit dimension), 0 if none
+  auto unit_label = [&](const std::vector<int> &labs) {
+    for (int l : labs)
+      if (ext[l] > 1) return l;
+    return 0;
+  };
+  const int l1 = unit_label(llab), r1 = unit_label(rlab);
+  // staging along k for an operand whose stride-1 index is contracted, else
+  // along its rows
+  const bool lkf = l1 != 0 && std::find(Kl.begin(), Kl.end(), l1) != Kl.end();
+  const bool rkf = r1 != 0 && std::find(Kr.begin(), Kr.end(), r1) != Kr.end();
+  // row order: the operand's own order when it is staged along rows (its
+  // stride-1 index first), else D's order (the result's rows and the split-K
+  // sum then write along D's stride-1 index)
+  auto by_d = [&](std::vector<int> &labs) {
+    std::stable_sort(labs.begin(), labs.end(), [&](int a, int b) { return sd[a] < sd[b]; });
+  };
+  if (lkf) by_d(M);
+  if (rkf) by_d(N);
+  for (int ko = 0; ko < 2; ++ko) {
+    const std::vector<int> &K = ko ? Kr : Kl;
+    if (ko && Kr == Kl) break;
+    for (int sw = 0; sw < 2; ++sw) {
+      ContractPlan c;
+      c.kase = 3;
+      c.fused = true;
+      c.swap = sw;
+      c.m = vol(M); c.n = vol(N); c.k = vol(K);
+      GatherGroup lrow, lk, rrow, rk, dm, dn;
+      if (!gather_group(M, ext, sl, lrow) || !gather_group(K, ext, sl, lk) || !gather_group(N, ext, sr, rrow) ||
+          !gather_group(K, ext, sr, rk) || !gather_group(M, ext, sd, dm) || !gather_group(N, ext, sd, dn))
+        return 0;
+      if (!sw) {
+        c.maps = GatherMaps{lrow, lk, rrow, rk, dm, dn};
+        c.a_kfast = lkf; c.b_kfast = rkf;
+      } else {
+        c.maps = GatherMaps{rrow, rk, lrow, lk, dn, dm};
+        c.a_kfast = rkf; c.b_kfast = lkf;
+      }
+      out.push_back(c);
+    }
+  }
   return 0;
 }
 
@@ -195,7 +313,7 @@ void put_plan(float *f, int kind, const std::vector<int> &dims, const std::vecto
 
 int plan_candidates(const std::vector<int> &dig, int dr, int lr, int rr, const std::vector<int> &dd,
                     const std::vector<int> &ld, const std::vector<int> &rd, int kind, R4Mode r4mode, bool acc,
-                    int num_cus, std::vector<PlanCandidate> &out)
+                    int num_cus, std::vector<PlanCandidate> &out, bool with_fused)
 {
   out.clear();
   const int es = kind_size(kind);
@@ -247,6 +365,18 @@ int plan_candidates(const std::vector<int> &dig, int dr, int lr, int rr, const s
       if (!c.d_out_o2n.empty()) put_plan(f + 42, kind, c.dmat_dims, c.d_out_o2n);
       out.push_back(pc);
     }
+  }
+  const bool split3 = (kind == R4 || kind == C4) && r4mode == R4Mode::kBF16x3;
+  if (with_fused && !split3) {
+    std::vector<ContractPlan> fz;
+    if (plan_fused(dig, dr, lr, rr, dd, ld, rd, fz) == 0)
+      for (int v = 0; v < (int)fz.size(); ++v)
+        for (int t = 0; t < ntiles; ++t) {
+          PlanCandidate pc;
+          pc.variant = kFusedVariantBase + v;
+          pc.tile = t;
+          out.push_back(pc);
+        }
   }
   return 0;
 }

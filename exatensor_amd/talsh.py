@@ -39,6 +39,7 @@ NOT_CLEAN = -192837465
 COPY_D, COPY_M, COPY_T, COPY_K = 0, 1, 2, 3
 COPY_MT, COPY_TT, COPY_KK, COPY_MM = 6, 10, 15, 5
 COPY_MTT, COPY_MMM, COPY_KKK, COPY_TTT, COPY_MKK = 26, 21, 63, 42, 31
+FUSED_VARIANT = 1000  # plan variants of the fused-gather candidates start here
 MAX_TENSOR_RANK = 56
 MAX_TENSOR_OPERANDS = 4
 NP_DTYPE = {R4: np.float32, R8: np.float64, C4: np.complex64, C8: np.complex128}
@@ -133,6 +134,7 @@ def load_library(path=LIB_PATH):
         "talshAmdAutotuneClear": (C.c_int, []),
         "talshAmdPlanCandidates": (C.c_int, [C.c_char_p, C.c_int, _IP, C.c_int, _IP, C.c_int, _IP, C.c_int, C.c_int,
                                              C.c_int, C.POINTER(C.c_double), _IP]),
+        "talshAmdPlanFusedCandidates": (C.c_int, [C.c_char_p, C.c_int, _IP, C.c_int, _IP, C.c_int, _IP, _IP]),
         "talshAmdDeviceSynchronize": (C.c_int, [C.c_int]),
         "talshAmdPermutePlan": (C.c_int, [C.c_int, C.c_int, _IP, _IP, _IP]),
         "talshAmdGemm": (C.c_int, [C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p, C.c_int,

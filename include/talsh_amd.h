@@ -27,13 +27,23 @@ int talshAmdProfileQuery(int kernel_class, double *total_ms, long long *launches
 
 /* Plan of subsequent talshTensorContract calls on GEMM-shaped patterns:
  * variant >= 0 selects the variant-th candidate matricization (ranked by the
- * built-in byte-count heuristic; 0 is the heuristic's choice), -1 restores
- * the heuristic; tile selects the GEMM tile shape (-1 default). */
+ * built-in byte-count heuristic; 0 is the heuristic's choice), variant
+ * TALSH_AMD_FUSED_VARIANT + i the i-th fused-gather candidate (no transposes:
+ * the matrix product reads L and R and updates D through index offset
+ * tables; native precision, so R4/C4 under fast math keep the heuristic's
+ * matricization), -1 restores the heuristic; tile selects the GEMM tile
+ * shape (-1 default). */
+#define TALSH_AMD_FUSED_VARIANT 1000
 int talshAmdSetPlanVariant(int variant, int tile);
 /* Number of distinct matricization candidates of a contraction and the bytes
  * each candidate moves in transposes (moved[] of capacity cap). */
 int talshAmdPlanCandidates(const char *cptrn, int drank, const int *ddims, int lrank, const int *ldims, int rrank,
                            const int *rdims, int data_kind, int accumulative, int cap, double *moved, int *ncand);
+/* Number of fused-gather candidates of a contraction (0 when an index group
+ * spans more than 16 dimensions after unit extents are dropped and contiguous
+ * runs merged). No counterpart in the reference, which always transposes. */
+int talshAmdPlanFusedCandidates(const char *cptrn, int drank, const int *ddims, int lrank, const int *ldims,
+                                int rrank, const int *rdims, int *nfused);
 
 /* Plan of the DLF permutation kernel for a tensor of data_kind with extents
  * dims[rank] and 0-based O2N permutation o2n[rank]: info[0] fused rank,
