@@ -466,8 +466,7 @@ bool autotune_choose(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens
   }
   if (!resident_on(ltens, d->flat_id) || !resident_on(rtens, d->flat_id)) return false;
   std::vector<PlanCandidate> cands;
-  if (plan_candidates(dig, dr, lr, rr, dd, ld, rd, kind, d->r4mode, acc, d->num_cus, cands, true) ||
-      cands.empty())
+  if (plan_candidates(dig, dr, lr, rr, dd, ld, rd, kind, d->r4mode, acc, d->num_cus, cands) || cands.empty())
     return false;
   talsh_tens_t scratch;
   talshTensorClean(&scratch);

@@ -52,7 +52,7 @@ int plan_contraction(const std::vector<int> &dig, int dr, int lr, int rr, const 
                      ContractPlan &P, std::vector<ContractPlan> *all);
 
 // ---- learned plan selector ---------------------------------------------------
-constexpr int kPlanFeatures = 44;
+constexpr int kPlanFeatures = 54;
 
 // One (matricization, GEMM tile) candidate and its feature vector.
 struct PlanCandidate {
@@ -61,11 +61,11 @@ struct PlanCandidate {
 };
 
 // Every candidate of a GEMM-shaped contraction with its features.
-// with_fused adds the fused-gather candidates (native precision only) after
-// the matricization candidates; the learned selector ranks only the latter.
+// The matricization candidates, then the fused-gather candidates (native
+// precision only: not for R4/C4 under the three-way split).
 int plan_candidates(const std::vector<int> &dig, int dr, int lr, int rr, const std::vector<int> &dd,
                     const std::vector<int> &ld, const std::vector<int> &rd, int kind, R4Mode r4mode, bool acc,
-                    int num_cus, std::vector<PlanCandidate> &out, bool with_fused = false);
+                    int num_cus, std::vector<PlanCandidate> &out);
 
 // Training domain of the selector (random contractions up to 2^35 flops and
 // operands of up to 2^26 elements, with margin); larger problems use the

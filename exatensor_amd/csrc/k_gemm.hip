@@ -1086,13 +1086,13 @@ int current_cus()
 // k-values (multiples of the 16-deep k-step) so that the grid fills the chip;
 // the partials are summed in FP64 by k_splitk_sum. Returns the number of
 // parts (1: no split) and the chunk length.
-int splitk_parts(const GemmArgs &g, int tile, int64_t *kchunk)
+int splitk_parts_cus(const GemmArgs &g, int tile, int cus, int64_t *kchunk)
 {
   *kchunk = (int64_t)g.k;
   const bool real = g.kind == R8 || (g.kind == R4 && g.r4mode == R4Mode::kNativeF32);
   if (!real || tile != 2 || g.k < 512) return 1;
   const int64_t tiles = (int64_t)((g.m + 63) / 64) * (int64_t)((g.n + 63) / 64);
-  const int64_t want = 2 * (int64_t)current_cus();
+  const int64_t want = 2 * (int64_t)cus;
   if (tiles >= want) return 1;
   int64_t nz = std::min<int64_t>({8, want / tiles, (int64_t)g.k / 256});
   if (nz < 2) return 1;
@@ -1100,6 +1100,8 @@ int splitk_parts(const GemmArgs &g, int tile, int64_t *kchunk)
   *kchunk = (chunk + 15) / 16 * 16;
   return (int)(((int64_t)g.k + *kchunk - 1) / *kchunk);
 }
+
+int splitk_parts(const GemmArgs &g, int tile, int64_t *kchunk) { return splitk_parts_cus(g, tile, current_cus(), kchunk); }
 
 // Large complex products in native precision run realified through the real
 // 128x128 kernels (which re-read operands from L2 half as often as the 64x64
@@ -1164,6 +1166,14 @@ int gemm_default_tile(int kind, R4Mode r4mode, size_t m, size_t n, size_t k)
   // otherwise 64x64 tiles balance better (measured over random contractions)
   const size_t tiles128 = ((m + 127) / 128) * ((n + 127) / 128);
   return (tiles128 >= (size_t)(2 * cus) && k >= 2048) ? 0 : 2;
+}
+
+int gemm_splitk_count(int kind, R4Mode r4mode, size_t m, size_t n, size_t k, int tile, int cus)
+{
+  GemmArgs g{};
+  g.kind = kind; g.r4mode = r4mode; g.m = m; g.n = n; g.k = k;
+  int64_t kc;
+  return splitk_parts_cus(g, tile, cus, &kc);
 }
 
 int gemm_tile_choices(int kind, R4Mode r4mode)

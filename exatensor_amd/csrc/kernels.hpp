@@ -101,6 +101,9 @@ int gemm_noalloc_scratch(void **ptr);
 // (bm x bn) and occupancy target of a choice.
 int gemm_tile_choices(int kind, R4Mode r4mode);
 void gemm_tile_shape(int kind, R4Mode r4mode, int tile, int *bm, int *bn, int *occupancy);
+// Number of K parts of a real product in native precision on tile choice
+// `tile` on a GPU with cus compute units (1: no split K).
+int gemm_splitk_count(int kind, R4Mode r4mode, size_t m, size_t n, size_t k, int tile, int cus);
 // Tile choice of launch_gemm when GemmArgs::tile < 0.
 int gemm_default_tile(int kind, R4Mode r4mode, size_t m, size_t n, size_t k);
 

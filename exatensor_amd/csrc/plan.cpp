@@ -313,7 +313,7 @@ void put_plan(float *f, int kind, const std::vector<int> &dims, const std::vecto
 
 int plan_candidates(const std::vector<int> &dig, int dr, int lr, int rr, const std::vector<int> &dd,
                     const std::vector<int> &ld, const std::vector<int> &rd, int kind, R4Mode r4mode, bool acc,
-                    int num_cus, std::vector<PlanCandidate> &out, bool with_fused)
+                    int num_cus, std::vector<PlanCandidate> &out)
 {
   out.clear();
   const int es = kind_size(kind);
@@ -360,6 +360,7 @@ int plan_candidates(const std::vector<int> &dig, int dr, int lr, int rr, const s
       f[36] = c.a_kfast && kal && !cplx;
       f[37] = c.b_kfast && kal && !cplx;
       // transpose kernel plans: vector width and the scatter fallback
+      f[53] = lg2(gemm_splitk_count(kind, r4mode, (size_t)gm, (size_t)gn, (size_t)gk, t, num_cus > 0 ? num_cus : 256));
       put_plan(f + 38, kind, ld, c.l_o2n);
       put_plan(f + 40, kind, rd, c.r_o2n);
       if (!c.d_out_o2n.empty()) put_plan(f + 42, kind, c.dmat_dims, c.d_out_o2n);
@@ -367,16 +368,51 @@ int plan_candidates(const std::vector<int> &dig, int dr, int lr, int rr, const s
     }
   }
   const bool split3 = (kind == R4 || kind == C4) && r4mode == R4Mode::kBF16x3;
-  if (with_fused && !split3) {
-    std::vector<ContractPlan> fz;
-    if (plan_fused(dig, dr, lr, rr, dd, ld, rd, fz) == 0)
-      for (int v = 0; v < (int)fz.size(); ++v)
-        for (int t = 0; t < ntiles; ++t) {
-          PlanCandidate pc;
-          pc.variant = kFusedVariantBase + v;
-          pc.tile = t;
-          out.push_back(pc);
-        }
+  std::vector<ContractPlan> fz;
+  if (split3 || plan_fused(dig, dr, lr, rr, dd, ld, rd, fz) != 0) return 0;
+  for (int v = 0; v < (int)fz.size(); ++v) {
+    const ContractPlan &c = fz[v];
+    const double gm = (double)(c.swap ? c.n : c.m), gn = (double)(c.swap ? c.m : c.n), gk = (double)c.k;
+    const GatherGroup &sa = c.a_kfast ? c.maps.a_k : c.maps.a_row, &sb = c.b_kfast ? c.maps.b_k : c.maps.b_row;
+    auto unit_run = [](const GatherGroup &g) { return (g.nd > 0 && g.str[0] == 1) ? (double)g.ext[0] : 1.0; };
+    auto first_ext = [](const GatherGroup &g) { return g.nd > 0 ? g.ext[0] : 1; };
+    for (int t = 0; t < ntiles; ++t) {
+      PlanCandidate pc;
+      pc.variant = kFusedVariantBase + v;
+      pc.tile = t;
+      float *f = pc.f;
+      f[0] = kind == R4; f[1] = kind == R8; f[2] = kind == C4; f[3] = kind == C8;
+      f[5] = acc;
+      f[6] = lg2(gm); f[7] = lg2(gn); f[8] = lg2(gk);
+      f[9] = lg2((cplx ? 8.0 : 2.0) * gm * gn * gk);
+      f[10] = c.a_kfast; f[11] = c.b_kfast; f[12] = c.swap;
+      f[13 + t] = 1.f;
+      f[17] = lg2((double)es * (gm * gk + gn * gk + gm * gn * (acc ? 2.0 : 1.0)));
+      int bm, bn, occ;
+      gemm_tile_shape(kind, r4mode, t, &bm, &bn, &occ);
+      const double tm = std::ceil(gm / bm), tn = std::ceil(gn / bn);
+      f[33] = lg2(tm * bm * tn * bn / (gm * gn));
+      f[34] = (float)std::log2(tm * tn / (double)(num_cus > 0 ? num_cus : 256));
+      f[35] = (float)occ;
+      // 16-byte groups along each operand's staging direction
+      f[36] = !cplx && unit_run(sa) * es >= 16 && ((int)unit_run(sa) * es) % 16 == 0;
+      f[37] = !cplx && unit_run(sb) * es >= 16 && ((int)unit_run(sb) * es) % 16 == 0;
+      f[44] = 1.f;  // fused gather
+      const int run = bm / 16;  // elements one thread stages along its direction
+      f[45] = first_ext(sa) % run == 0 && first_ext(sb) % run == 0;
+      f[46] = c.maps.c_row.nd > 0 && c.maps.c_row.str[0] == 1;  // result rows along D's stride-1 index
+      // contiguous run along each operand's staging direction, and the
+      // stride between its staged elements when that is not 1
+      f[47] = lg2(unit_run(sa));
+      f[48] = lg2(unit_run(sb));
+      f[49] = lg2(sa.nd > 0 ? (double)sa.str[0] : 1.0);
+      f[50] = lg2(sb.nd > 0 ? (double)sb.str[0] : 1.0);
+      // runs of the result's rows and columns along D's stride-1 index
+      f[51] = lg2(unit_run(c.maps.c_row));
+      f[52] = lg2(unit_run(c.maps.c_col));
+      f[53] = lg2(gemm_splitk_count(kind, r4mode, (size_t)gm, (size_t)gn, (size_t)gk, t, num_cus > 0 ? num_cus : 256));
+      out.push_back(pc);
+    }
   }
   return 0;
 }

@@ -164,15 +164,20 @@ def load_dataset(path):
     return specs, times
 
 
-def design_matrix(lib, specs, times):
-    """Stacked features and log-time labels of every candidate; group offsets."""
+def design_matrix(lib, specs, times, clip=None):
+    """Stacked features and log-time labels of every candidate; group offsets.
+    clip: labels of candidates slower than clip x the contraction's fastest
+    are capped there (the model need not rank hopeless candidates precisely)."""
     X, y, groups = [], [], [0]
     for spec, tm in zip(specs, times):
         _, _, f = plan_features(lib, spec)
         if len(f) != len(tm):
             raise RuntimeError(f"candidate count changed for {spec['pattern']}")
         X.append(f)
-        y.append(np.log(np.maximum(tm, 1e-4)))
+        t = np.maximum(tm, 1e-4)
+        if clip:
+            t = np.minimum(t, clip * t.min())
+        y.append(np.log(t))
         groups.append(groups[-1] + len(tm))
     return np.concatenate(X).astype(np.float32), np.concatenate(y).astype(np.float32), groups
 
@@ -297,6 +302,7 @@ def main(argv=None):
     t.add_argument("--epochs", type=int, default=60)
     t.add_argument("--batch", type=int, default=512)
     t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--clip", type=float, default=None)
     ev = sub.add_parser("evaluate")
     ev.add_argument("--data", nargs="+", required=True)
     ev.add_argument("--model", default=DEFAULT_MODEL)
@@ -314,7 +320,7 @@ def main(argv=None):
         specs += s_
         times += t_
     if a.cmd == "train":
-        X, y, _ = design_matrix(lib, specs, times)
+        X, y, _ = design_matrix(lib, specs, times, a.clip)
         m = Mlp(lib, X.shape[1], a.hidden, a.width)
         t0 = time.time()
         loss = m.train(X, y, a.epochs, a.batch, a.lr)

@@ -43,14 +43,23 @@ def test_candidate_features(lib):
         assert set(t.tolist()) == set(range(ntiles))
         # one-hot data kind and tile
         assert np.all(f[:, 0:4].sum(1) == 1) and np.all(f[:, 13:16].sum(1) == 1)
+        # matricization candidates first, then the fused-gather ones (flagged,
+        # no transpose bytes)
+        fused = v >= T.FUSED_VARIANT
+        assert np.array_equal(f[:, 44] == 1, fused) and np.all(f[fused, 16] == 0)
+        assert not fused[0] and np.all(np.diff(fused.astype(int)) >= 0)
         # variant 0 is the heuristic's choice: fewest transpose bytes
-        assert f[v == 0, 16].min() <= f[:, 16].min() + 1e-6
+        assert f[v == 0, 16].min() <= f[~fused, 16].min() + 1e-6
         moved = (C.c_double * 64)()
-        nc = C.c_int()
+        nc, nfz = C.c_int(), C.c_int()
         e = lib.talshAmdPlanCandidates(s["pattern"].encode(), len(s["ddims"]), T.ints(s["ddims"]), len(s["ldims"]),
                                        T.ints(s["ldims"]), len(s["rdims"]), T.ints(s["rdims"]), s["kind"],
                                        T.YEP if s["acc"] else T.NOPE, 64, moved, C.byref(nc))
-        assert e == 0 and nc.value == len(set(v.tolist()))
+        assert e == 0 and nc.value == len(set(v[~fused].tolist()))
+        assert lib.talshAmdPlanFusedCandidates(s["pattern"].encode(), len(s["ddims"]), T.ints(s["ddims"]),
+                                               len(s["ldims"]), T.ints(s["ldims"]), len(s["rdims"]),
+                                               T.ints(s["rdims"]), C.byref(nfz)) == 0
+        assert nfz.value == len(set(v[fused].tolist())) and 1 <= nfz.value <= 4
 
 
 def test_model_file_roundtrip_and_host_choice(lib, tmp_path):
