@@ -234,16 +234,31 @@ def run_contractions(gb, reps=3):
     pat1 = b"D(a,b,c,d)+=L(d,i,b,j)*R(j,c,i,a)"
     d = gb.tensor(T.R8, [32] * 4); l = gb.tensor(T.R8, [32] * 4); r = gb.tensor(T.R8, [32] * 4)
     gb.fill(l, 3); gb.fill(r, 4)
-    gb.L.talshTensorContract(pat1, d.ptr, l.ptr, r.ptr, 1.0, 0.0, gb.gpu, T.DEV_NULL, T.COPY_MTT, T.YEP, None)
-    gb.sync()
     n1 = 20
-    t0 = time.perf_counter()
-    for _ in range(n1):
+
+    def time_c1():
         gb.L.talshTensorContract(pat1, d.ptr, l.ptr, r.ptr, 1.0, 0.0, gb.gpu, T.DEV_NULL, T.COPY_MTT, T.YEP, None)
-    gb.sync()
-    t1 = time.perf_counter()
-    out["c1_r8_ext32"] = dict(tflops=round(2.0 * 32 ** 6 * n1 / (t1 - t0) / 1e12, 3),
-                              ms_per_op=round((t1 - t0) / n1 * 1e3, 3))
+        gb.sync()
+        t0 = time.perf_counter()
+        for _ in range(n1):
+            gb.L.talshTensorContract(pat1, d.ptr, l.ptr, r.ptr, 1.0, 0.0, gb.gpu, T.DEV_NULL, T.COPY_MTT, T.YEP, None)
+        gb.sync()
+        return (time.perf_counter() - t0) / n1
+
+    dt = time_c1()
+    v, t = C.c_int(), C.c_int()
+    gb.L.talshAmdSelectorChoice(pat1, 4, T.ints([32] * 4), 4, T.ints([32] * 4), 4, T.ints([32] * 4), T.R8, T.YEP, 0,
+                                C.byref(v), C.byref(t))
+    # the same calls planned by the transpose-only byte-count heuristic
+    mode = gb.L.talshAmdGetPlanMode()
+    gb.L.talshAmdSetPlanMode(0)
+    try:
+        dt0 = time_c1()
+    finally:
+        gb.L.talshAmdSetPlanMode(mode)
+    out["c1_r8_ext32"] = dict(tflops=round(2.0 * 32 ** 6 / dt / 1e12, 3), ms_per_op=round(dt * 1e3, 3),
+                              plan=dict(variant=v.value, tile=t.value, fused=v.value >= T.FUSED_VARIANT),
+                              heuristic_ms_per_op=round(dt0 * 1e3, 3))
     for t in (d, l, r):
         t.destruct()
     return out

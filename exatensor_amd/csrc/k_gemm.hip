@@ -67,6 +67,7 @@ struct GemmParams {
   int64_t a_ks0, b_ks0;      // stride of the first contracted dimension in A / B (gather)
   int64_t a_rs0, b_rs0;      // stride of the first row dimension of A / B (gather)
   const void *zpad;          // 64 zero bytes, 16-byte aligned (gather)
+  int part_rowmajor;         // split-K partials m x n row-major (else column-major)
 };
 
 // Maps a workgroup to an output tile: first gather the workgroups that the
@@ -423,7 +424,8 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
       for (int r = 0; r < 4; ++r) {
         const int64_t row = m0 + wm + i * 16 + (sizeof(T) == 8 ? fk + 4 * r : fk * 4 + r);
         off[j][r] = (row < p.m && col < p.n)
-                        ? (split ? row + col * p.m : GATHER ? grow[r] + gcol[j] : row + col * p.ldc)
+                        ? (split ? (p.part_rowmajor ? row * p.n + col : row + col * p.m)
+                                 : GATHER ? grow[r] + gcol[j] : row + col * p.ldc)
                         : -1;
       }
     }
@@ -916,17 +918,22 @@ __global__ __launch_bounds__(256) void k_splitk_sum(const T *__restrict__ part, 
   }
 }
 
-// Split-K sum of a gathered product: the partials are m x n column-major,
-// C element (r, c) sits at c_row[r] + c_col[c].
+// Split-K sum of a gathered product: the partials are m x n, column-major or
+// (rowmajor) row-major; C element (r, c) sits at c_row[r] + c_col[c]. The
+// launcher makes the partials row-major when D's stride-1 index leads the
+// columns, so both the product's epilogue (16 consecutive columns per row)
+// and this sum write along it.
 template <typename T>
-__global__ __launch_bounds__(256) void k_splitk_sum_gather(const T *__restrict__ part, int nz, int64_t m, int64_t mn,
-                                                           const int64_t *__restrict__ c_row,
+__global__ __launch_bounds__(256) void k_splitk_sum_gather(const T *__restrict__ part, int nz, int64_t m, int64_t n,
+                                                           int rowmajor, const int64_t *__restrict__ c_row,
                                                            const int64_t *__restrict__ c_col, T *__restrict__ c,
                                                            double alpha, int beta_zero)
 {
-  const int64_t st = (int64_t)gridDim.x * 256;
+  const int64_t st = (int64_t)gridDim.x * 256, mn = m * n;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < mn; i += st) {
-    const int64_t col = i / m, row = i - col * m;
+    int64_t row, col;
+    if (rowmajor) { row = i / n; col = i - row * n; }
+    else { col = i / m; row = i - col * m; }
     const int64_t off = c_row[row] + c_col[col];
     double v = 0.0;
     for (int z = 0; z < nz; ++z) v += (double)part[(int64_t)z * mn + i];
@@ -1402,6 +1409,7 @@ int launch_gemm_gather(const GemmArgs &g, const GatherMaps &mp, void *ws, size_t
     if (nz > 1 && ws_bytes >= tb + (size_t)nz * (size_t)mn * es) {
       p.part = static_cast<char *>(ws) + tb;
       p.kchunk = kc;
+      p.part_rowmajor = mp.c_col.nd > 0 && mp.c_col.str[0] == 1;
       const unsigned rb = (unsigned)std::min<int64_t>((mn + 255) / 256, 4096);
       int e;
       if (g.kind == R8) {
@@ -1409,14 +1417,14 @@ int launch_gemm_gather(const GemmArgs &g, const GatherMaps &mp, void *ws, size_t
                     : launch_family<double, false, 64, 64, 16, 2, 2>(p, g.a_kfast, g.b_kfast, s);
         if (!e)
           hipLaunchKernelGGL(k_splitk_sum_gather<double>, dim3(rb), dim3(256), 0, s,
-                             static_cast<const double *>(p.part), nz, p.m, mn, p.c_row, p.c_col,
+                             static_cast<const double *>(p.part), nz, p.m, p.n, p.part_rowmajor, p.c_row, p.c_col,
                              static_cast<double *>(g.c), g.alpha_re, p.beta_zero);
       } else {
         e = runs(4) ? launch_family<float, false, 64, 64, 16, 2, 1>(p, g.a_kfast, g.b_kfast, s)
                     : launch_family<float, false, 64, 64, 16, 2, 2>(p, g.a_kfast, g.b_kfast, s);
         if (!e)
           hipLaunchKernelGGL(k_splitk_sum_gather<float>, dim3(rb), dim3(256), 0, s,
-                             static_cast<const float *>(p.part), nz, p.m, mn, p.c_row, p.c_col,
+                             static_cast<const float *>(p.part), nz, p.m, p.n, p.part_rowmajor, p.c_row, p.c_col,
                              static_cast<float *>(g.c), g.alpha_re, p.beta_zero);
       }
       if (e) return e;
