@@ -106,8 +106,9 @@ struct Frag4<double> { typedef double4 type; };
 
 // GATHER: 0 matricized operands; 1 fused gather where every thread's staged
 // elements stay inside one run of the staging direction's first dimension
-// (offsets by stride arithmetic from one table entry); 2 fused gather with a
-// table lookup per staged element or 16-byte group.
+// (offsets by stride arithmetic from one table entry) and real operands load
+// in vectors; 2 fused gather with a table lookup per staged element or
+// vector.
 template <typename T, bool CPLX, bool AK, bool BKF, int BM, int BN, int BK, int OCC, int GATHER = 0>
 __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
 {
@@ -133,13 +134,19 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
   const int64_t kbeg = split ? (int64_t)blockIdx.y * p.kchunk : 0;
   const int64_t kend = split ? min(p.k, kbeg + p.kchunk) : p.k;
 
-  // staging coordinates: k-fast operands: a thread covers LA consecutive k of
-  // one row; row-fast operands: LA consecutive rows of one k.
+  // staging coordinates. k-fast operands: a thread covers LA consecutive k of
+  // one row. Row-fast operands: RV consecutive rows (one vector of two real
+  // elements; one complex element) at each of KV = L / RV consecutive k, so a
+  // wave's loads of one k are contiguous along rows and each row's KV values
+  // go to LDS in one store. Registers: element (row + e, k + j) of a row-fast
+  // operand is r[.][j * RV + e].
+  constexpr int RV = CPLX ? 1 : 2;
+  constexpr int KVA = LA / RV, KVB = LB / RV;
   int a_r, a_k, b_r, b_k;
   if (AK) { a_r = tid / (BK / LA); a_k = (tid % (BK / LA)) * LA; }
-  else { a_k = tid / (BM / LA); a_r = (tid % (BM / LA)) * LA; }
+  else { a_r = (tid % (BM / RV)) * RV; a_k = (tid / (BM / RV)) * KVA; }
   if (BKF) { b_r = tid / (BK / LB); b_k = (tid % (BK / LB)) * LB; }
-  else { b_k = tid / (BN / LB); b_r = (tid % (BN / LB)) * LB; }
+  else { b_r = (tid % (BN / RV)) * RV; b_k = (tid / (BN / RV)) * KVB; }
 
   // fused gather: offset of the thread's first staged row (loop invariant)
   int64_t ga_row = 0, gb_row = 0;
@@ -147,18 +154,19 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
     ga_row = p.a_row[m0 + a_r < p.m ? m0 + a_r : 0];
     gb_row = p.b_row[n0 + b_r < p.n ? n0 + b_r : 0];
   }
-  // One operand element (or 16-byte group) of a gather load; lanes out of
-  // range (ok == false) read a zero block instead. The load itself is
+  // One operand element (or vector of N elements) of a gather load; lanes out
+  // of range (ok == false) read a zero block instead. The load itself is
   // unconditional and its value is used as loaded: a lane-masked load, or a
   // select on the loaded value, makes the compiler wait for the loads in
   // flight right there instead of at the LDS store.
   constexpr int VW = 16 / (int)sizeof(T);
-  typedef T TV __attribute__((ext_vector_type(16 / sizeof(T))));
   const T *zpad = static_cast<const T *>(p.zpad);
-  auto gather_vec = [&](const T *X, int64_t off, bool ok, T *dst) {
-    const TV v = *reinterpret_cast<const TV *>(ok ? X + off : zpad);
+  auto gather_vec = [&](const T *X, int64_t off, bool ok, T *dst, auto n) {
+    constexpr int N = decltype(n)::value;
+    typedef T TN __attribute__((ext_vector_type(N)));
+    const TN v = *reinterpret_cast<const TN *>(ok ? X + off : zpad);
 #pragma unroll
-    for (int e = 0; e < VW; ++e) dst[e] = v[e];
+    for (int e = 0; e < N; ++e) dst[e] = v[e];
   };
   auto gather_one = [&](const T *X, int64_t off, bool ok, int conj, T &re, T &im) {
     if (CPLX) {
@@ -171,24 +179,30 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
     }
   };
   // Gather staging of one operand tile, kf: along k (row offset fixed), else
-  // along rows (one k per thread). kbase is the table entry of the thread's
-  // first k (loaded one tile ahead), row_base the offset of its first row.
-  // GATHER == 1: a thread's L consecutive k (rows) stay inside one run of the
-  // first contracted (row) dimension, stride ks0 (rs0) (the launcher checks
-  // that the extent is a multiple of L); GATHER == 2: each k (row) looks up
-  // its own table entry.
+  // along rows. kbase is the table entry of the thread's first k (loaded one
+  // tile ahead), row_base the offset of its first row. GATHER == 1: a
+  // thread's consecutive k (rows) stay inside one run of the first contracted
+  // (row) dimension, stride ks0 (rs0) (the launcher checks the extents);
+  // GATHER == 2: each k (row) looks up its own table entry. vec: 16-byte (k
+  // direction) or RV-element (row direction) groups are contiguous.
   auto gather_load = [&](const T *X, const int64_t *rowtab, int64_t row_base, int64_t rs0, const int64_t *kof,
                          int64_t kbase, int64_t ks0, int64_t rows, int64_t row0,
                          int k_off, int64_t k0, bool vec, int conj, auto kf, auto dst) {
     constexpr int L = (int)(sizeof(dst[0]) / sizeof(T));
     constexpr bool RUNS = GATHER == 1;
+    // GATHER == 1 kernels of real kinds always take the vector path (the
+    // launcher checks it applies): a run-time choice between vector and
+    // scalar loads made the compiler wait for one operand's loads before
+    // issuing the other's
+    if (RUNS && !CPLX) vec = true;
     const int64_t kk0 = k0 + k_off;
     if (decltype(kf)::value) {
       const bool rok = row0 < rows;
       if (RUNS) {
         if (!CPLX && vec) {
 #pragma unroll
-          for (int i = 0; i < L; i += VW) gather_vec(X, row_base + kbase + i, rok && kk0 + i < kend, &dst[0][i]);
+          for (int i = 0; i < L; i += VW)
+            gather_vec(X, row_base + kbase + i, rok && kk0 + i < kend, &dst[0][i], std::integral_constant<int, VW>());
         } else {
 #pragma unroll
           for (int i = 0; i < L; ++i)
@@ -198,7 +212,7 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
 #pragma unroll
         for (int i = 0; i < L; i += VW) {
           const bool ok = rok && kk0 + i < kend;
-          gather_vec(X, row_base + kof[ok ? kk0 + i : 0], ok, &dst[0][i]);
+          gather_vec(X, row_base + kof[ok ? kk0 + i : 0], ok, &dst[0][i], std::integral_constant<int, VW>());
         }
       } else {
 #pragma unroll
@@ -208,27 +222,22 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
         }
       }
     } else {
-      const bool kok = kk0 < kend;
-      if (RUNS) {
-        if (!CPLX && vec) {
+      constexpr int KV = L / RV;
 #pragma unroll
-          for (int i = 0; i < L; i += VW) gather_vec(X, row_base + i + kbase, kok && row0 + i < rows, &dst[0][i]);
+      for (int j = 0; j < KV; ++j) {
+        const bool kok = kk0 + j < kend;
+        const int64_t ko = RUNS ? kbase + j * ks0 : (j == 0 ? kbase : kof[kok ? kk0 + j : 0]);
+        if (!CPLX && vec) {
+          // rows come in contiguous groups of RV (the row extent is a multiple)
+          const int64_t ro = RUNS ? row_base : rowtab[row0 < rows ? row0 : 0];
+          gather_vec(X, ro + ko, kok && row0 < rows, &dst[0][j * RV], std::integral_constant<int, RV>());
         } else {
 #pragma unroll
-          for (int i = 0; i < L; ++i)
-            gather_one(X, row_base + i * rs0 + kbase, kok && row0 + i < rows, conj, dst[0][i], dst[1 % NP][i]);
-        }
-      } else if (!CPLX && vec) {
-#pragma unroll
-        for (int i = 0; i < L; i += VW) {
-          const bool ok = kok && row0 + i < rows;
-          gather_vec(X, rowtab[ok ? row0 + i : 0] + kbase, ok, &dst[0][i]);
-        }
-      } else {
-#pragma unroll
-        for (int i = 0; i < L; ++i) {
-          const bool ok = kok && row0 + i < rows;
-          gather_one(X, rowtab[ok ? row0 + i : 0] + kbase, ok, conj, dst[0][i], dst[1 % NP][i]);
+          for (int e = 0; e < RV; ++e) {
+            const bool ok = kok && row0 + e < rows;
+            const int64_t ro = RUNS ? row_base + e * rs0 : rowtab[ok ? row0 + e : 0];
+            gather_one(X, ro + ko, ok, conj, dst[0][j * RV + e], dst[1 % NP][j * RV + e]);
+          }
         }
       }
     }
@@ -240,81 +249,120 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
   };
   int64_t ka_next = 0, kb_next = 0;  // gather: entries of the next tile to load
 
-  T ra[NP][LA], rb[NP][LB];
-  auto load_a = [&](int64_t k0) {
-    if (GATHER) {
-      gather_load(A, p.a_row, ga_row, p.a_rs0, p.a_kof, ka_next, p.a_ks0, p.m, m0 + a_r, a_k, k0,
-                  p.vec_a != 0, p.conj_a, std::integral_constant<bool, AK>(), ra);
-      return;
-    }
-    const bool vec = !CPLX && AK && interior_mn && (k0 + BK <= kend) && p.vec_a;
-    if (vec) {
-      const T *src = A + (m0 + a_r) * p.lda + k0 + a_k;
+  // Matricized staging of one operand tile: X stored rows x k with leading
+  // dimension ld (k fastest when kf, rows fastest otherwise); vec: 16-byte
+  // aligned rows (kf) or columns.
+  auto mat_load = [&](const T *X, int64_t ld, int64_t rows, int64_t row0, int k_off, int64_t k0, bool vec,
+                      bool interior, int conj, auto kf, auto dst) {
+    constexpr int L = (int)(sizeof(dst[0]) / sizeof(T));
+    const int64_t kk0 = k0 + k_off;
+    if (decltype(kf)::value) {
+      if (!CPLX && vec && interior && k0 + BK <= kend) {
+        const T *src = X + row0 * ld + kk0;
 #pragma unroll
-      for (int i = 0; i < LA; i += 16 / (int)sizeof(T)) *reinterpret_cast<float4 *>(&ra[0][i]) = *reinterpret_cast<const float4 *>(src + i);
-      return;
-    }
+        for (int i = 0; i < L; i += VW) {
+          typedef T TV __attribute__((ext_vector_type(16 / sizeof(T))));
+          const TV v = *reinterpret_cast<const TV *>(src + i);
 #pragma unroll
-    for (int i = 0; i < LA; ++i) {
-      const int64_t r = m0 + a_r + (AK ? 0 : i), kk = k0 + a_k + (AK ? i : 0);
-      const bool ok = r < p.m && kk < kend;
-      const int64_t off = AK ? (r * p.lda + kk) : (kk * p.lda + r);
-      if (CPLX) {
-        ra[0][i] = ok ? A[2 * off] : T(0);
-        const T im = ok ? A[2 * off + 1] : T(0);
-        ra[1 % NP][i] = p.conj_a ? -im : im;
-      } else {
-        ra[0][i] = ok ? A[off] : T(0);
+          for (int e = 0; e < VW; ++e) dst[0][i + e] = v[e];
+        }
+        return;
       }
+#pragma unroll
+      for (int i = 0; i < L; ++i) {
+        const int64_t kk = kk0 + i;
+        const bool ok = row0 < rows && kk < kend;
+        const int64_t off = row0 * ld + kk;
+        if (CPLX) {
+          dst[0][i] = ok ? X[2 * off] : T(0);
+          const T im = ok ? X[2 * off + 1] : T(0);
+          dst[1 % NP][i] = conj ? -im : im;
+        } else {
+          dst[0][i] = ok ? X[off] : T(0);
+        }
+      }
+    } else {
+      constexpr int KV = L / RV;
+      if (!CPLX && vec && interior && k0 + BK <= kend) {
+        typedef T TR __attribute__((ext_vector_type(RV)));
+#pragma unroll
+        for (int j = 0; j < KV; ++j) {
+          const TR v = *reinterpret_cast<const TR *>(X + (kk0 + j) * ld + row0);
+#pragma unroll
+          for (int e = 0; e < RV; ++e) dst[0][j * RV + e] = v[e];
+        }
+        return;
+      }
+#pragma unroll
+      for (int j = 0; j < KV; ++j)
+#pragma unroll
+        for (int e = 0; e < RV; ++e) {
+          const int64_t kk = kk0 + j, r = row0 + e;
+          const bool ok = r < rows && kk < kend;
+          const int64_t off = kk * ld + r;
+          if (CPLX) {
+            dst[0][j * RV + e] = ok ? X[2 * off] : T(0);
+            const T im = ok ? X[2 * off + 1] : T(0);
+            dst[1 % NP][j * RV + e] = conj ? -im : im;
+          } else {
+            dst[0][j * RV + e] = ok ? X[off] : T(0);
+          }
+        }
     }
   };
+
+  T ra[NP][LA], rb[NP][LB];
+  auto load_a = [&](int64_t k0) {
+    if (GATHER)
+      gather_load(A, p.a_row, ga_row, p.a_rs0, p.a_kof, ka_next, p.a_ks0, p.m, m0 + a_r, a_k, k0,
+                  p.vec_a != 0, p.conj_a, std::integral_constant<bool, AK>(), ra);
+    else
+      mat_load(A, p.lda, p.m, m0 + a_r, a_k, k0, p.vec_a != 0, interior_mn, p.conj_a,
+               std::integral_constant<bool, AK>(), ra);
+  };
   auto load_b = [&](int64_t k0) {
-    if (GATHER) {
+    if (GATHER)
       gather_load(B, p.b_row, gb_row, p.b_rs0, p.b_kof, kb_next, p.b_ks0, p.n, n0 + b_r, b_k, k0,
                   p.vec_b != 0, p.conj_b, std::integral_constant<bool, BKF>(), rb);
-      return;
-    }
-    const bool vec = !CPLX && BKF && interior_mn && (k0 + BK <= kend) && p.vec_b;
-    if (vec) {
-      const T *src = B + (n0 + b_r) * p.ldb + k0 + b_k;
+    else
+      mat_load(B, p.ldb, p.n, n0 + b_r, b_k, k0, p.vec_b != 0, interior_mn, p.conj_b,
+               std::integral_constant<bool, BKF>(), rb);
+  };
+  // LDS store of one operand's staged tile: k-fast rows in 16-byte pieces;
+  // row-fast: each of the RV rows gets its KV consecutive k in one store.
+  auto lds_store = [&](auto S, const auto &r, int row, int k, auto kf) {
+    constexpr int L = (int)(sizeof(r[0]) / sizeof(T));
 #pragma unroll
-      for (int i = 0; i < LB; i += 16 / (int)sizeof(T)) *reinterpret_cast<float4 *>(&rb[0][i]) = *reinterpret_cast<const float4 *>(src + i);
-      return;
-    }
+    for (int q = 0; q < NP; ++q) {
+      if (decltype(kf)::value) {
 #pragma unroll
-    for (int i = 0; i < LB; ++i) {
-      const int64_t r = n0 + b_r + (BKF ? 0 : i), kk = k0 + b_k + (BKF ? i : 0);
-      const bool ok = r < p.n && kk < kend;
-      const int64_t off = BKF ? (r * p.ldb + kk) : (kk * p.ldb + r);
-      if (CPLX) {
-        rb[0][i] = ok ? B[2 * off] : T(0);
-        const T im = ok ? B[2 * off + 1] : T(0);
-        rb[1 % NP][i] = p.conj_b ? -im : im;
+        for (int i = 0; i < L; i += VW) {
+          typedef T TV __attribute__((ext_vector_type(16 / sizeof(T))));
+          TV v;
+#pragma unroll
+          for (int e = 0; e < VW; ++e) v[e] = r[q][i + e];
+          *reinterpret_cast<TV *>(&S[q][row][k + i]) = v;
+        }
       } else {
-        rb[0][i] = ok ? B[off] : T(0);
+        constexpr int KV = L / RV;
+#pragma unroll
+        for (int e = 0; e < RV; ++e) {
+          if constexpr (KV == 1) {
+            S[q][row + e][k] = r[q][e];
+          } else {
+            typedef T TK __attribute__((ext_vector_type(KV)));
+            TK v;
+#pragma unroll
+            for (int j = 0; j < KV; ++j) v[j] = r[q][j * RV + e];
+            *reinterpret_cast<TK *>(&S[q][row + e][k]) = v;
+          }
+        }
       }
     }
   };
   auto store = [&](int buf) {
-#pragma unroll
-    for (int q = 0; q < NP; ++q) {
-      if (AK) {
-#pragma unroll
-        for (int i = 0; i < LA; i += 16 / (int)sizeof(T))
-          *reinterpret_cast<float4 *>(&As[buf][q][a_r][a_k + i]) = *reinterpret_cast<const float4 *>(&ra[q][i]);
-      } else {
-#pragma unroll
-        for (int i = 0; i < LA; ++i) As[buf][q][a_r + i][a_k] = ra[q][i];
-      }
-      if (BKF) {
-#pragma unroll
-        for (int i = 0; i < LB; i += 16 / (int)sizeof(T))
-          *reinterpret_cast<float4 *>(&Bs[buf][q][b_r][b_k + i]) = *reinterpret_cast<const float4 *>(&rb[q][i]);
-      } else {
-#pragma unroll
-        for (int i = 0; i < LB; ++i) Bs[buf][q][b_r + i][b_k] = rb[q][i];
-      }
-    }
+    lds_store(As[buf], ra, a_r, a_k, std::integral_constant<bool, AK>());
+    lds_store(Bs[buf], rb, b_r, b_k, std::integral_constant<bool, BKF>());
   };
 
   AccT acc[NP][FM][FN];
@@ -1395,11 +1443,17 @@ int launch_gemm_gather(const GemmArgs &g, const GatherMaps &mp, void *ws, size_t
   p.b_ks0 = mp.b_k.nd ? mp.b_k.str[0] : 0;
   p.a_rs0 = mp.a_row.nd ? mp.a_row.str[0] : 0;
   p.b_rs0 = mp.b_row.nd ? mp.b_row.str[0] : 0;
-  // every thread's staged run (8 elements on 128x128 tiles, 4 on 64x64) stays
-  // inside one run of the staging direction's first dimension
+  // every thread's staged elements (L = 8 on 128x128 tiles, 4 on 64x64) stay
+  // inside runs of the first dimensions: L consecutive k of a k-fast operand;
+  // RV consecutive rows (2 real, 1 complex) at L / RV consecutive k of a
+  // row-fast one
   auto runs = [&](int L) {
-    const GatherGroup &sa = g.a_kfast ? mp.a_k : mp.a_row, &sb = g.b_kfast ? mp.b_k : mp.b_row;
-    return (sa.nd ? sa.ext[0] : 1) % L == 0 && (sb.nd ? sb.ext[0] : 1) % L == 0;
+    const int rv = cplx ? 1 : 2;
+    auto e0 = [](const GatherGroup &q) { return q.nd ? q.ext[0] : 1; };
+    auto ok = [&](bool kf, const GatherGroup &row, const GatherGroup &k) {
+      return kf ? e0(k) % L == 0 : (e0(row) % rv == 0 && e0(k) % (L / rv) == 0);
+    };
+    return ok(g.a_kfast, mp.a_row, mp.a_k) && ok(g.b_kfast, mp.b_row, mp.b_k) && (cplx || (p.vec_a && p.vec_b));
   };
   const int tile = g.tile < 0 ? gemm_default_tile(g.kind, g.r4mode, g.m, g.n, g.k) : g.tile;
   if (!cplx) {
