@@ -474,17 +474,21 @@ bool autotune_choose(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens
   std::vector<std::pair<hipEvent_t, hipEvent_t>> sink;
   double best = -1.0;
   int bv = -1, bt = -1;
-  // fused-gather candidates are left out above the selector's flop domain:
-  // there the transposes are a small share of the time and the matricized
-  // product is the faster kernel (measured over 20 000 random contractions:
-  // the fused plan wins 83% of them at 2^25 flops, 28% at 2^34, 14% at 2^36)
+  // Above the selector's flop domain, fused-gather candidates are left out
+  // when their operands are not both staged along a contiguous run (each
+  // staged element is then a separate access: 2.5-7 times the matricized
+  // product's time on the 96/128 contraction) and for complex kinds (a large
+  // complex product runs faster realified than in the complex kernel the
+  // gather uses: 61 against 57 TFLOP/s in FP64 there).
   double lflops = 1.0;  // log2 of the real flop count, as the selector's domain is defined
   for (int i = 0; i < lr; ++i) lflops += std::log2((double)ld[i]);
   for (int j = 0; j < rr; ++j)
     if (dig[lr + j] > 0) lflops += std::log2((double)rd[j]);
-  const bool skip_fused = lflops > kSelectorMaxLog2Flops;
+  const bool large = lflops > kSelectorMaxLog2Flops;
   for (const PlanCandidate &c : cands) {
-    if (skip_fused && c.variant >= kFusedVariantBase) continue;
+    // features 47 and 48: log2 of A's and B's contiguous run along their staging direction
+    if (large && c.variant >= kFusedVariantBase && (kind_complex(kind) || c.f[47] <= 0.f || c.f[48] <= 0.f))
+      continue;
     double tmin = -1.0;
     for (int rep = 0; rep < 3; ++rep) {
       prof_set_sink(&sink);
