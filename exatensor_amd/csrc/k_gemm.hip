@@ -252,12 +252,14 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
   // Matricized staging of one operand tile: X stored rows x k with leading
   // dimension ld (k fastest when kf, rows fastest otherwise); vec: 16-byte
   // aligned rows (kf) or columns.
+  // force: the caller has checked that the vector form applies.
   auto mat_load = [&](const T *X, int64_t ld, int64_t rows, int64_t row0, int k_off, int64_t k0, bool vec,
-                      bool interior, int conj, auto kf, auto dst) {
+                      bool interior, int conj, auto kf, auto dst, auto force) {
     constexpr int L = (int)(sizeof(dst[0]) / sizeof(T));
+    constexpr bool F = decltype(force)::value;
     const int64_t kk0 = k0 + k_off;
     if (decltype(kf)::value) {
-      if (!CPLX && vec && interior && k0 + BK <= kend) {
+      if (F || (!CPLX && vec && interior && k0 + BK <= kend)) {
         const T *src = X + row0 * ld + kk0;
 #pragma unroll
         for (int i = 0; i < L; i += VW) {
@@ -283,7 +285,7 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
       }
     } else {
       constexpr int KV = L / RV;
-      if (!CPLX && vec && interior && k0 + BK <= kend) {
+      if (F || (!CPLX && vec && interior && k0 + BK <= kend)) {
         typedef T TR __attribute__((ext_vector_type(RV)));
 #pragma unroll
         for (int j = 0; j < KV; ++j) {
@@ -318,7 +320,7 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
                   p.vec_a != 0, p.conj_a, std::integral_constant<bool, AK>(), ra);
     else
       mat_load(A, p.lda, p.m, m0 + a_r, a_k, k0, p.vec_a != 0, interior_mn, p.conj_a,
-               std::integral_constant<bool, AK>(), ra);
+               std::integral_constant<bool, AK>(), ra, std::false_type());
   };
   auto load_b = [&](int64_t k0) {
     if (GATHER)
@@ -326,7 +328,21 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
                   p.vec_b != 0, p.conj_b, std::integral_constant<bool, BKF>(), rb);
     else
       mat_load(B, p.ldb, p.n, n0 + b_r, b_k, k0, p.vec_b != 0, interior_mn, p.conj_b,
-               std::integral_constant<bool, BKF>(), rb);
+               std::integral_constant<bool, BKF>(), rb, std::false_type());
+  };
+  // Both operands of a tile: when both take the vector form, their loads are
+  // issued back to back in one branch (separate per-operand choices made the
+  // compiler wait for A's loads at the join before issuing B's).
+  auto load_ab = [&](int64_t k0) {
+    if (!GATHER && !CPLX && p.vec_a && p.vec_b && interior_mn && k0 + BK <= kend) {
+      mat_load(A, p.lda, p.m, m0 + a_r, a_k, k0, true, true, 0, std::integral_constant<bool, AK>(), ra,
+               std::true_type());
+      mat_load(B, p.ldb, p.n, n0 + b_r, b_k, k0, true, true, 0, std::integral_constant<bool, BKF>(), rb,
+               std::true_type());
+      return;
+    }
+    load_a(k0);
+    load_b(k0);
   };
   // LDS store of one operand's staged tile: k-fast rows in 16-byte pieces;
   // row-fast: each of the RV rows gets its KV consecutive k in one store.
@@ -378,8 +394,7 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
     ka_next = kof_at(p.a_kof, kbeg, a_k);
     kb_next = kof_at(p.b_kof, kbeg, b_k);
   }
-  load_a(kbeg);
-  load_b(kbeg);
+  load_ab(kbeg);
   if (GATHER) {  // table entries one tile ahead of the operand loads
     ka_next = kof_at(p.a_kof, kbeg + BK, a_k);
     kb_next = kof_at(p.b_kof, kbeg + BK, b_k);
@@ -390,8 +405,7 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_mfma(const GemmParams p)
   for (int64_t kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nk) {
-      load_a(kbeg + (kt + 1) * BK);
-      load_b(kbeg + (kt + 1) * BK);
+      load_ab(kbeg + (kt + 1) * BK);
       if (GATHER) {
         ka_next = kof_at(p.a_kof, kbeg + (kt + 2) * BK, a_k);
         kb_next = kof_at(p.b_kof, kbeg + (kt + 2) * BK, b_k);
