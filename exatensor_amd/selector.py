@@ -3,9 +3,11 @@ GPU training and regret evaluation.
 
 The engine's planner enumerates every distinct matricization of a GEMM-shaped
 contraction (contracted order from L or R, free orders from the operand or
-from D, D or D^T) times every GEMM tile shape of the data kind. The selector
-is a small MLP that predicts log(kernel time) of each candidate from its
-features (talshAmdPlanFeatures) and picks the minimum. It replaces fixed rules
+from D, D or D^T) and every fused-gather plan (no transposes: the product
+reads the tensors through offset tables), times every GEMM tile shape of the
+data kind. The selector is a small MLP that predicts log(kernel time) of each
+candidate from its features (talshAmdPlanFeatures) and picks the minimum, so
+it chooses the operand order, fused vs. separate transposes and the tile. It replaces fixed rules
 of the kind the reference hard-codes (TALSH/talshc.cpp:366-488,
 TALSH/tensor_algebra_gpu_nvidia.cu:1188-1272, TALSH/tensor_algebra_cpu.F90:9291-9298).
 
