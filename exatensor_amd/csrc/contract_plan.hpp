@@ -1,5 +1,6 @@
-// Contraction planning: matricization candidates of a GEMM-shaped tensor
-// contraction, their features, and the learned plan selector.
+// Contraction planning: matricization and fused-gather candidates of a
+// GEMM-shaped tensor contraction, their features, and the learned plan
+// selector.
 #pragma once
 
 #include <string>

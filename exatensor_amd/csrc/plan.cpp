@@ -1,15 +1,17 @@
-// Contraction planning (TTGT matricization candidates), their features, and
-// the learned plan selector that picks among them.
+// Contraction planning (TTGT matricization candidates and fused-gather
+// candidates), their features, and the learned plan selector that picks
+// among them.
 //
 // The reference fixes one matricization, D(ll x lr) += L^T(lc x ll) R(lc x lr)
 // with every operand transposed into that order
 // (TALSH/tensor_algebra_cpu.F90:5613-5755, get_contr_permutations), and fixed
 // heuristics elsewhere (TALSH/tensor_algebra_gpu_nvidia.cu:1188-1272,
 // TALSH/talshc.cpp:366-488). Here the GEMM reads either operand k-fast or
-// m/n-fast and may compute D^T, so several candidates exist; the byte-count
-// heuristic ranks them and an optional learned model (a small MLP predicting
-// the log of kernel time from the features below) chooses among candidate and
-// GEMM tile shape.
+// m/n-fast and may compute D^T, and a fused-gather plan reads the tensors
+// where they lie through offset tables, so several candidates exist; the
+// byte-count heuristic ranks the transposing ones and an optional learned
+// model (a small MLP predicting the log of kernel time from the features
+// below) chooses among all candidates and GEMM tile shapes.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
