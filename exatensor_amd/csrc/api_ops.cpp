@@ -1796,6 +1796,33 @@ int talshAmdPlanFusedCandidates(const char *cptrn, int drank, const int *ddims, 
   return 0;
 }
 
+int talshAmdPlanFusedMaps(const char *cptrn, int drank, const int *ddims, int lrank, const int *ldims, int rrank,
+                          const int *rdims, int candidate, int *info, int *ext, long long *str)
+{
+  if (!cptrn || !info || !ext || !str) return TALSH_INVALID_ARGS;
+  int digc[2 * MAX_TENSOR_RANK], dr, lr, rr, cj;
+  if (talsh_get_contr_ptrn_str2dig(cptrn, digc, &dr, &lr, &rr, &cj)) return TALSH_INVALID_ARGS;
+  if (dr != drank || lr != lrank || rr != rrank || dr == 0 || lr == 0 || rr == 0) return TALSH_INVALID_ARGS;
+  std::vector<int> dig(digc, digc + lr + rr), dd(ddims, ddims + dr), ld(ldims, ldims + lr), rd(rdims, rdims + rr);
+  std::vector<ContractPlan> fz;
+  const int e = plan_fused(dig, dr, lr, rr, dd, ld, rd, fz);
+  if (e) return e;
+  if (candidate < 0 || candidate >= (int)fz.size()) return TALSH_INVALID_ARGS;
+  const ContractPlan &c = fz[candidate];
+  const GatherGroup *g[6] = {&c.maps.a_row, &c.maps.a_k, &c.maps.b_row, &c.maps.b_k, &c.maps.c_row, &c.maps.c_col};
+  info[0] = c.swap ? 1 : 0;
+  info[1] = c.a_kfast ? 1 : 0;
+  info[2] = c.b_kfast ? 1 : 0;
+  for (int t = 0; t < 6; ++t) {
+    info[3 + t] = g[t]->nd;
+    for (int j = 0; j < kGatherMaxDims; ++j) {
+      ext[kGatherMaxDims * t + j] = j < g[t]->nd ? g[t]->ext[j] : 1;
+      str[kGatherMaxDims * t + j] = j < g[t]->nd ? (long long)g[t]->str[j] : 0;
+    }
+  }
+  return 0;
+}
+
 int talshAmdPermutePlan(int data_kind, int rank, const int *dims, const int *o2n, int *info)
 {
   if (!dims || !o2n || !info) return TALSH_INVALID_ARGS;

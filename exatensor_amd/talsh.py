@@ -135,6 +135,8 @@ def load_library(path=LIB_PATH):
         "talshAmdPlanCandidates": (C.c_int, [C.c_char_p, C.c_int, _IP, C.c_int, _IP, C.c_int, _IP, C.c_int, C.c_int,
                                              C.c_int, C.POINTER(C.c_double), _IP]),
         "talshAmdPlanFusedCandidates": (C.c_int, [C.c_char_p, C.c_int, _IP, C.c_int, _IP, C.c_int, _IP, _IP]),
+        "talshAmdPlanFusedMaps": (C.c_int, [C.c_char_p, C.c_int, _IP, C.c_int, _IP, C.c_int, _IP, C.c_int, _IP, _IP,
+                                            C.POINTER(C.c_longlong)]),
         "talshAmdDeviceSynchronize": (C.c_int, [C.c_int]),
         "talshAmdPermutePlan": (C.c_int, [C.c_int, C.c_int, _IP, _IP, _IP]),
         "talshAmdGemm": (C.c_int, [C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p, C.c_int,

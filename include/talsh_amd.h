@@ -44,6 +44,15 @@ int talshAmdPlanCandidates(const char *cptrn, int drank, const int *ddims, int l
  * runs merged). No counterpart in the reference, which always transposes. */
 int talshAmdPlanFusedCandidates(const char *cptrn, int drank, const int *ddims, int lrank, const int *ldims,
                                 int rrank, const int *rdims, int *nfused);
+/* Index groups of fused-gather candidate `candidate` (for inspection and host
+ * tests): info[0] 1 if the product computes D^T (A = R, B = L), info[1] and
+ * info[2] 1 if A / B are staged along k, info[3 + g] the number of dimensions
+ * of group g in the order A rows, A k, B rows, B k, C rows, C columns; ext[16 g
+ * + j] and str[16 g + j] the extent and element stride of dimension j of
+ * group g (element i of a group sits at sum_j digit_j(i) * str[j], digits of
+ * i over ext[], first fastest). No counterpart in the reference. */
+int talshAmdPlanFusedMaps(const char *cptrn, int drank, const int *ddims, int lrank, const int *ldims, int rrank,
+                          const int *rdims, int candidate, int *info, int *ext, long long *str);
 
 /* Plan of the DLF permutation kernel for a tensor of data_kind with extents
  * dims[rank] and 0-based O2N permutation o2n[rank]: info[0] fused rank,
